@@ -1,0 +1,423 @@
+// fiberhip.hip -- C ABI of libfiberhip (see include/fiberhip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fiberhip.h"
+#include "fh_single.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define FH_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(e_ == hipErrorOutOfMemory ? FH_ENOMEM : FH_EDEVICE,                \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+}  // namespace
+
+struct fh_context_s {
+    int device;
+    hipStream_t stream;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct fh_plan_s {
+    fh_context ctx;
+    fh_problem_desc d;
+    int threads;  // workgroup size of the single-mode kernel
+    DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
+    size_t work_per_solve = 0;  // complex entries
+    int64_t total_bytes = 0;
+    double gamma0 = 0.0;        // gamma[1,1,1,1] of a single-mode plan
+};
+
+namespace {
+
+int dev_alloc(fh_plan p, DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? FH_ENOMEM : FH_EDEVICE,
+                    std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    p->total_bytes += (int64_t)bytes;
+    return FH_OK;
+}
+
+void dev_free(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <int N, int T> void fill_twiddles(std::vector<std::complex<long double>>& out)
+{
+    using PL = fh::FftPlan<N, T>;
+    out.assign(PL::TW_SIZE, std::complex<long double>(1, 0));
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int q = 0; q < PL::NPASS; ++q) {
+        const int rad = 1 << PL::log_radix(q), ns = 1 << PL::log_ns(q);
+        const int off = PL::tw_offset(q);
+        for (int r = 1; r < rad; ++r)
+            for (int jm = 0; jm < ns; ++jm) {
+                long double a = -two_pi * (long double)jm * (long double)r / ((long double)ns * rad);
+                out[off + (r - 1) * ns + jm] = std::complex<long double>(cosl(a), sinl(a));
+            }
+    }
+}
+
+// ------------------------------------------------------------ dispatch
+
+// Workgroup size of the single-mode kernel per precision and length: 8
+// samples per thread for FP64 (1024 threads from 8192 samples up) keeps the
+// radix-8 working set in registers; FP32 uses 16 from 4096 samples up.
+template <typename R, int N> struct SingleShape {
+    static constexpr int T = sizeof(R) == 8 ? (N >= 8192 ? 1024 : (N >= 256 ? N / 8 : N / 4))
+                                            : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4));
+};
+
+template <typename R, int N> struct SingleKernel {
+    static constexpr int T = SingleShape<R, N>::T;
+    using LY = fh::SmLayout<R, N, T>;
+    static size_t work_per_solve() { return (size_t)(LY::NW > 0 ? LY::NW : 1) * N; }
+    static void twiddles(std::vector<std::complex<long double>>& tw) { fill_twiddles<N, T>(tw); }
+    static hipError_t launch(const fh::SmArgs<R>& a, int batch, hipStream_t s)
+    {
+        hipLaunchKernelGGL((fh::k_rk4ip_single<R, N, T>), dim3(batch), dim3(T), 0, s, a);
+        return hipGetLastError();
+    }
+};
+
+template <typename R, template <typename, int> class Op, typename... A>
+auto with_size(int nt, A&&... args) -> decltype(Op<R, 1024>::run(args...))
+{
+    switch (nt) {
+    case 64: return Op<R, 64>::run(args...);
+    case 128: return Op<R, 128>::run(args...);
+    case 256: return Op<R, 256>::run(args...);
+    case 512: return Op<R, 512>::run(args...);
+    case 1024: return Op<R, 1024>::run(args...);
+    case 2048: return Op<R, 2048>::run(args...);
+    case 4096: return Op<R, 4096>::run(args...);
+    case 8192: return Op<R, 8192>::run(args...);
+    case 16384: return Op<R, 16384>::run(args...);
+    default: return Op<R, 1024>::unsupported();
+    }
+}
+
+template <typename R, int N> struct OpWork {
+    static size_t run() { return SingleKernel<R, N>::work_per_solve(); }
+    static size_t unsupported() { return 0; }
+};
+template <typename R, int N> struct OpTw {
+    static int run(std::vector<std::complex<long double>>& tw)
+    {
+        SingleKernel<R, N>::twiddles(tw);
+        return SingleKernel<R, N>::T;
+    }
+    static int unsupported() { return 0; }
+};
+template <typename R, int N> struct OpLaunch {
+    static hipError_t run(const fh::SmArgs<R>& a, int batch, hipStream_t s)
+    {
+        return SingleKernel<R, N>::launch(a, batch, s);
+    }
+    static hipError_t unsupported() { return hipErrorInvalidValue; }
+};
+
+// -------------------------------------------------------- energy kernel
+
+// One workgroup per (field, mode): FP64 sum of |u|^2 in a fixed tree order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_energy(const fh::C2<R>* __restrict__ u, int nt, double* __restrict__ e)
+{
+    __shared__ double red[256];
+    const fh::C2<R>* col = u + (size_t)blockIdx.x * nt;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nt; t += 256) {
+        const double x = (double)col[t].x, y = (double)col[t].y;
+        s += x * x + y * y;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) e[blockIdx.x] = red[0];
+}
+
+int check_desc(const fh_problem_desc* d)
+{
+    if (!d) return fail(FH_EINVAL, "problem descriptor is NULL");
+    if (!is_pow2(d->nt) || d->nt < 4) return fail(FH_EINVAL, "nt must be a power of two >= 4");
+    if (d->modes < 1) return fail(FH_EINVAL, "modes must be positive");
+    if (d->precision != FH_PREC_FP64 && d->precision != FH_PREC_FP32_BULK)
+        return fail(FH_EINVAL, "unknown precision");
+    if (d->scheme != FH_SCHEME_RK4IP_FIXED) return fail(FH_EINVAL, "unknown scheme");
+    if (d->steps <= 0) return fail(FH_EINVAL, "steps must be a positive integer");
+    if (d->max_batch <= 0) return fail(FH_EINVAL, "max_batch must be positive");
+    if (!(std::isfinite(d->length_m) && d->length_m > 0)) return fail(FH_EINVAL, "fiber length must be positive and finite");
+    if (!d->dispersion || !d->gamma || !d->raman || !d->shock) return fail(FH_EINVAL, "operator array is NULL");
+    const size_t nt = (size_t)d->nt, m = (size_t)d->modes;
+    for (size_t i = 0; i < nt * m; ++i)
+        if (!std::isfinite(d->dispersion[i])) return fail(FH_EINVAL, "dispersion must be finite");
+    for (size_t i = 0; i < m * m * m * m; ++i)
+        if (!std::isfinite(d->gamma[i])) return fail(FH_EINVAL, "gamma tensor must be finite");
+    for (size_t i = 0; i < 2 * nt; ++i)
+        if (!std::isfinite(d->raman[i])) return fail(FH_EINVAL, "Raman response must be finite");
+    for (size_t i = 0; i < nt; ++i) {
+        if (!std::isfinite(d->shock[i])) return fail(FH_EINVAL, "shock factor must be finite");
+        if (!(d->shock[i] > 0))
+            return fail(FH_EGRID, "simulation bandwidth reaches nonpositive absolute optical frequencies "
+                                  "(shock factor w/w0 <= 0); increase the time window or use fewer samples");
+    }
+    if (!std::isfinite(d->one_m_fr) || d->one_m_fr < 0 || d->one_m_fr > 1)
+        return fail(FH_EINVAL, "one_m_fR must lie in [0, 1]");
+    double dmax = 0.0;
+    for (size_t i = 0; i < nt * m; ++i) dmax = std::max(dmax, std::fabs(d->dispersion[i]));
+    if (dmax * d->length_m >= FH_MAX_PHASE)
+        return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
+    if (d->modes != 1) return fail(FH_EUNSUPPORTED, "this build integrates single-mode problems only");
+    if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
+    return FH_OK;
+}
+
+template <typename R> int upload_tables(fh_plan p)
+{
+    const fh_problem_desc& d = p->d;
+    const int nt = d.nt;
+    std::vector<std::complex<long double>> tw;
+    const int T = with_size<R, OpTw>(nt, tw);
+    if (T == 0) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(nt));
+    p->threads = T;
+    std::vector<R> twr(2 * tw.size()), hn(2 * (size_t)nt), shn(nt);
+    for (size_t i = 0; i < tw.size(); ++i) {
+        twr[2 * i] = (R)tw[i].real();
+        twr[2 * i + 1] = (R)tw[i].imag();
+    }
+    for (int k = 0; k < nt; ++k) {
+        const double sgn = (k & 1) ? -1.0 : 1.0;
+        hn[2 * k] = (R)(d.raman[2 * k] * sgn / nt);
+        hn[2 * k + 1] = (R)(d.raman[2 * k + 1] * sgn / nt);
+        shn[k] = (R)(d.shock[k] / nt);
+    }
+    int rc;
+    if ((rc = dev_alloc(p, p->disp, sizeof(double) * (size_t)nt * d.modes))) return rc;
+    if ((rc = dev_alloc(p, p->hn, sizeof(R) * hn.size()))) return rc;
+    if ((rc = dev_alloc(p, p->shn, sizeof(R) * shn.size()))) return rc;
+    if ((rc = dev_alloc(p, p->tw, sizeof(R) * twr.size()))) return rc;
+    FH_HIP(hipMemcpy(p->disp.p, d.dispersion, p->disp.bytes, hipMemcpyHostToDevice));
+    FH_HIP(hipMemcpy(p->hn.p, hn.data(), p->hn.bytes, hipMemcpyHostToDevice));
+    FH_HIP(hipMemcpy(p->shn.p, shn.data(), p->shn.bytes, hipMemcpyHostToDevice));
+    FH_HIP(hipMemcpy(p->tw.p, twr.data(), p->tw.bytes, hipMemcpyHostToDevice));
+    p->work_per_solve = with_size<R, OpWork>(nt);
+    const size_t mb = (size_t)d.max_batch;
+    if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
+    if ((rc = dev_alloc(p, p->status, sizeof(int) * mb))) return rc;
+    const size_t field = 2 * sizeof(R) * (size_t)nt * d.modes * mb;
+    if ((rc = dev_alloc(p, p->u_in, field))) return rc;
+    if ((rc = dev_alloc(p, p->u_out, field))) return rc;
+    return FH_OK;
+}
+
+template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+{
+    const fh_problem_desc& d = p->d;
+    fh::SmArgs<R> a;
+    a.u0 = (const fh::C2<R>*)d_u0;
+    a.out = (fh::C2<R>*)d_uL;
+    a.work = (fh::C2<R>*)p->work.p;
+    a.disp = (const double*)p->disp.p;
+    a.hn = (const fh::C2<R>*)p->hn.p;
+    a.shn = (const R*)p->shn.p;
+    a.tw = (const fh::C2<R>*)p->tw.p;
+    a.status = (int*)p->status.p;
+    a.gamma = (R)p->gamma0;
+    a.one_m_fr = (R)d.one_m_fr;
+    a.h = d.length_m / d.steps;
+    a.length = d.length_m;
+    a.steps = d.steps;
+    FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    hipError_t e = with_size<R, OpLaunch>(d.nt, a, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+int fh_context_create(int device, fh_context* out)
+{
+    if (!out) return fail(FH_EINVAL, "out is NULL");
+    int n = 0;
+    FH_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(FH_EINVAL, "no such device");
+    FH_HIP(hipSetDevice(device));
+    fh_context c = new fh_context_s;
+    c->device = device;
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(FH_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return FH_OK;
+}
+
+int fh_context_destroy(fh_context ctx)
+{
+    if (!ctx) return FH_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FH_OK;
+}
+
+int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
+{
+    if (!ctx || !out) return fail(FH_EINVAL, "context or out is NULL");
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    FH_HIP(hipSetDevice(ctx->device));
+    fh_plan p = new fh_plan_s;
+    p->ctx = ctx;
+    p->d = *desc;  // array pointers are only read during creation
+    p->gamma0 = desc->gamma[0];
+    rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p) : upload_tables<float>(p);
+    p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
+    if (rc) {
+        fh_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return FH_OK;
+}
+
+int fh_plan_destroy(fh_plan p)
+{
+    if (!p) return FH_OK;
+    (void)hipSetDevice(p->ctx->device);
+    for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out}) dev_free(*b);
+    delete p;
+    return FH_OK;
+}
+
+int fh_plan_device_bytes(fh_plan p, int64_t* bytes)
+{
+    if (!p || !bytes) return fail(FH_EINVAL, "plan or bytes is NULL");
+    *bytes = p->total_bytes;
+    return FH_OK;
+}
+
+int fh_forward_device(fh_plan p, int32_t batch, const void* d_u0, void* d_uL, void* stream)
+{
+    if (!p || !d_u0 || !d_uL) return fail(FH_EINVAL, "plan or field pointer is NULL");
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
+    return p->d.precision == FH_PREC_FP64 ? launch_forward<double>(p, batch, d_u0, d_uL, s)
+                                          : launch_forward<float>(p, batch, d_u0, d_uL, s);
+}
+
+int fh_forward(fh_plan p, int32_t batch, const double* u0, double* uL, fh_stats* stats)
+{
+    if (!p || !u0 || !uL) return fail(FH_EINVAL, "plan or field pointer is NULL");
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    const size_t n = (size_t)batch * p->d.nt * p->d.modes;
+    double norm2 = 0.0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");
+        norm2 += u0[i] * u0[i];
+    }
+    if (!(norm2 > 0)) return fail(FH_EINVAL, "input spectrum must be nonzero");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    if (fp64) {
+        FH_HIP(hipMemcpyAsync(p->u_in.p, u0, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    } else {
+        std::vector<float> tmp(2 * n);
+        for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)u0[i];
+        FH_HIP(hipMemcpyAsync(p->u_in.p, tmp.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+        FH_HIP(hipStreamSynchronize(s));
+    }
+    int rc = fp64 ? launch_forward<double>(p, batch, p->u_in.p, p->u_out.p, s)
+                  : launch_forward<float>(p, batch, p->u_in.p, p->u_out.p, s);
+    if (rc) return rc;
+    std::vector<int> status(batch);
+    FH_HIP(hipMemcpyAsync(status.data(), p->status.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    if (fp64) {
+        FH_HIP(hipMemcpyAsync(uL, p->u_out.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<float> tmp(2 * n);
+        FH_HIP(hipMemcpyAsync(tmp.data(), p->u_out.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < 2 * n; ++i) uL[i] = tmp[i];
+    }
+    int bad = 0;
+    for (int b = 0; b < batch; ++b) bad |= status[b];
+    if (stats) {
+        stats->accepted = p->d.steps;
+        stats->rejected = 0;
+        stats->rhs_evals = 4LL * p->d.steps;
+        stats->retcode = bad ? FH_ENONFINITE : FH_OK;
+    }
+    if (bad) return fail(FH_ENONFINITE, "fixed-step propagation produced non-finite fields");
+    return FH_OK;
+}
+
+int fh_energy_device(fh_plan p, int32_t batch, const void* d_u, double* energies)
+{
+    if (!p || !d_u || !energies) return fail(FH_EINVAL, "NULL argument");
+    if (batch < 1) return fail(FH_EINVAL, "batch must be positive");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const int cols = batch * p->d.modes;
+    double* d_e = nullptr;
+    FH_HIP(hipMallocAsync((void**)&d_e, sizeof(double) * cols, s));
+    if (p->d.precision == FH_PREC_FP64)
+        hipLaunchKernelGGL(k_energy<double>, dim3(cols), dim3(256), 0, s, (const fh::C2<double>*)d_u, p->d.nt, d_e);
+    else
+        hipLaunchKernelGGL(k_energy<float>, dim3(cols), dim3(256), 0, s, (const fh::C2<float>*)d_u, p->d.nt, d_e);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipMemcpyAsync(energies, d_e, sizeof(double) * cols, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipFreeAsync(d_e, s));
+    FH_HIP(hipStreamSynchronize(s));
+    return FH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+"""ctypes binding of libfiberhip (include/fiberhip.h).
+
+The shared library is built in-tree next to this package
+(``fiber-raman-suppression_amd/libfiberhip.so``).  There is no fallback: if
+the library is missing, every entry point raises ``FiberHIPUnavailable``.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR.parent / "libfiberhip.so"
+HEADER_PATH = PKG_DIR.parent.parent / "include" / "fiberhip.h"
+
+FH_OK = 0
+FH_EINVAL = 1
+FH_EGRID = 2
+FH_ENOTSYM = 3
+FH_ENONFINITE = 4
+FH_ENOMEM = 5
+FH_EDEVICE = 6
+FH_EUNSUPPORTED = 7
+
+FH_PREC_FP64 = 0
+FH_PREC_FP32_BULK = 1
+FH_SCHEME_RK4IP_FIXED = 0
+
+_NAMES = {
+    FH_EINVAL: "FH_EINVAL", FH_EGRID: "FH_EGRID", FH_ENOTSYM: "FH_ENOTSYM",
+    FH_ENONFINITE: "FH_ENONFINITE", FH_ENOMEM: "FH_ENOMEM", FH_EDEVICE: "FH_EDEVICE",
+    FH_EUNSUPPORTED: "FH_EUNSUPPORTED",
+}
+
+
+class FiberHIPUnavailable(RuntimeError):
+    """libfiberhip.so is not built or cannot be loaded."""
+
+
+class FiberHIPError(RuntimeError):
+    """A libfiberhip entry point returned an error code."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [
+        ("nt", ctypes.c_int32),
+        ("modes", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("scheme", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("length_m", ctypes.c_double),
+        ("dispersion", ctypes.POINTER(ctypes.c_double)),
+        ("gamma", ctypes.POINTER(ctypes.c_double)),
+        ("raman", ctypes.POINTER(ctypes.c_double)),
+        ("shock", ctypes.POINTER(ctypes.c_double)),
+        ("one_m_fr", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_int64),
+        ("rejected", ctypes.c_int64),
+        ("rhs_evals", ctypes.c_int64),
+        ("retcode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_LIB = None
+
+_VP = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_SIGNATURES = {
+    "fh_context_create": ([ctypes.c_int, ctypes.POINTER(_VP)], ctypes.c_int),
+    "fh_context_destroy": ([_VP], ctypes.c_int),
+    "fh_plan_create": ([_VP, ctypes.POINTER(ProblemDesc), ctypes.POINTER(_VP)], ctypes.c_int),
+    "fh_plan_destroy": ([_VP], ctypes.c_int),
+    "fh_plan_device_bytes": ([_VP, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
+    "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
+    "fh_last_error": ([], ctypes.c_char_p),
+}
+
+
+def exported_names() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def lib() -> ctypes.CDLL:
+    """Load libfiberhip.so (raises FiberHIPUnavailable when it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise FiberHIPUnavailable(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        try:
+            handle = ctypes.CDLL(str(LIB_PATH))
+        except OSError as exc:  # pragma: no cover - depends on the ROCm runtime
+            raise FiberHIPUnavailable(f"cannot load {LIB_PATH}: {exc}") from exc
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = handle
+    return _LIB
+
+
+def check(code: int) -> None:
+    if code != FH_OK:
+        msg = lib().fh_last_error()
+        raise FiberHIPError(code, msg.decode() if msg else "")

@@ -1,0 +1,141 @@
+"""Contexts and plans over the libfiberhip C ABI."""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import FH_PREC_FP32_BULK, FH_PREC_FP64, FH_SCHEME_RK4IP_FIXED, check, lib
+
+PRECISIONS = {"fp64": FH_PREC_FP64, "fp32": FH_PREC_FP32_BULK, "fp32_bulk": FH_PREC_FP32_BULK}
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """One HIP device and its stream (fh_context_create)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().fh_context_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            lib().fh_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def _column_major(a, dtype) -> np.ndarray:
+    """(nt, m, ...) array -> contiguous column-major copy (t fastest)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).T)
+
+
+class Plan:
+    """An immutable propagation plan (fh_plan_create) for one resolved fiber.
+
+    dispersion (nt, m) rad/m, gamma (m, m, m, m), raman (nt,) complex hR(ω),
+    shock (nt,) = fftshift(ωs/ω0), all in FFT order as the reference builds them.
+    """
+
+    def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: int,
+                 precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None):
+        self.context = context or default_context()
+        D = np.asarray(dispersion, dtype=np.float64)
+        if D.ndim == 1:
+            D = D[:, None]
+        self.nt, self.modes = D.shape
+        self.precision = precision
+        self.steps = int(steps)
+        self.length_m = float(length_m)
+        self.max_batch = int(max_batch)
+        self._D = _column_major(D, np.float64)
+        g = np.asarray(gamma, dtype=np.float64).reshape((self.modes,) * 4)
+        self._g = np.ascontiguousarray(g.transpose(3, 2, 1, 0))
+        self._hr = np.ascontiguousarray(np.asarray(raman, dtype=np.complex128).reshape(-1))
+        self._sh = np.ascontiguousarray(np.asarray(shock, dtype=np.float64).reshape(-1))
+        desc = _lib.ProblemDesc(
+            nt=self.nt, modes=self.modes, precision=PRECISIONS[precision], scheme=FH_SCHEME_RK4IP_FIXED,
+            steps=self.steps, max_batch=self.max_batch, length_m=self.length_m,
+            dispersion=_dp(self._D), gamma=_dp(self._g), raman=_dp(self._hr.view(np.float64)),
+            shock=_dp(self._sh), one_m_fr=float(one_m_fr))
+        h = ctypes.c_void_p()
+        check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def from_problem(cls, problem, steps: int, precision: str = "fp64", max_batch: int = 1,
+                     context: Optional[Context] = None) -> "Plan":
+        f = problem.fiber
+        from .problem import self_steepening
+
+        return cls(f["Dω"], f["γ"], f["hRω"], self_steepening(problem.sim), f["one_m_fR"], f["L"], steps,
+                   precision=precision, max_batch=max_batch, context=context)
+
+    @property
+    def device_bytes(self) -> int:
+        n = ctypes.c_int64()
+        check(lib().fh_plan_device_bytes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def forward(self, u0) -> tuple[np.ndarray, dict]:
+        """Batched forward solve.  u0: (nt, m) or (batch, nt, m) complex launch
+        spectra; returns lab-frame outputs of the same shape and the stats."""
+        u = np.asarray(u0, dtype=np.complex128)
+        single = u.ndim == 2
+        if single:
+            u = u[None]
+        if u.ndim != 3 or u.shape[1:] != (self.nt, self.modes):
+            raise ValueError(f"launch has shape {u.shape}, expected (batch, {self.nt}, {self.modes})")
+        batch = u.shape[0]
+        buf = np.ascontiguousarray(np.transpose(u, (0, 2, 1)))  # [batch][mode][t]
+        out = np.empty_like(buf)
+        st = _lib.Stats()
+        check(lib().fh_forward(self.handle, batch, _dp(buf.view(np.float64)), _dp(out.view(np.float64)),
+                               ctypes.byref(st)))
+        res = np.transpose(out, (0, 2, 1))
+        stats = dict(accepted=st.accepted, rejected=st.rejected, rhs_evaluations=st.rhs_evals,
+                     retcode=st.retcode)
+        return (res[0] if single else res), stats
+
+    def forward_device(self, batch: int, d_u0: int, d_uL: int, stream: int = 0) -> None:
+        """Enqueue a forward solve on device-resident fields (pointers as ints)."""
+        check(lib().fh_forward_device(self.handle, int(batch), ctypes.c_void_p(d_u0), ctypes.c_void_p(d_uL),
+                                      ctypes.c_void_p(stream) if stream else None))
+
+    def energy_device(self, batch: int, d_u: int) -> np.ndarray:
+        out = np.zeros(batch * self.modes)
+        check(lib().fh_energy_device(self.handle, int(batch), ctypes.c_void_p(d_u), _dp(out)))
+        return out.reshape(batch, self.modes)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().fh_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

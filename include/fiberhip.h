@@ -1,0 +1,128 @@
+/*
+ * fiberhip.h -- C ABI of libfiberhip, the MI355X (gfx950) GMMNLSE propagation
+ * and adjoint engine behind FiberLab's propagation seam.
+ *
+ * The reference (Nacho2027/fiber-raman-suppression, Julia) has no FFI; the
+ * replaceable calls are
+ *   solve_disp_mmf(uw0, fiber, sim)          src/simulation/simulate_disp_mmf.jl:183-207
+ *   solve_adjoint_disp_mmf(lwL, sol, f, s)   src/simulation/sensitivity_disp_mmf.jl:309-332
+ *   _scalar_rk4_output(problem; steps)       src/fiberlab/scalar_crosscheck.jl:8-68
+ * and the operator arrays they consume are built by
+ *   get_disp_sim_params / get_disp_fiber_params_user_defined /
+ *   _single_oscillator_raman_fields          src/helpers/helpers.jl:56-84,245-282,97-130
+ *   get_p_disp_mmf (selfsteep)               src/simulation/simulate_disp_mmf.jl:87-114
+ * A Julia caller binds the entry points below with `ccall` (see
+ * INTEGRATION.md); arrays are Julia-native column-major, so they pass without
+ * copies: a field of nt samples and m modes is m*nt complex doubles stored as
+ * interleaved (re, im) pairs, sample t of mode k at element k*nt + t, and a
+ * batch of fields is batch such blocks back to back.
+ *
+ * All functions return FH_OK (0) or one of the FH_E* codes; the message of
+ * the last failure on the calling thread is returned by fh_last_error().
+ */
+#ifndef FIBERHIP_H
+#define FIBERHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  Each mirrors a check of the reference:
+ *   FH_EINVAL     shape / finiteness / argument checks
+ *                 (propagation.jl:25-32, physics_models.jl:459-577)
+ *   FH_EGRID      non-positive absolute optical frequency on the grid
+ *                 (helpers.jl:14-22: shock factor w/w0 must stay > 0)
+ *   FH_ENOTSYM    adjoint needs a permutation-symmetric gamma
+ *                 (sensitivity_disp_mmf.jl:103-112,239-240)
+ *   FH_ENONFINITE solver produced NaN/Inf (scalar_crosscheck.jl:66,
+ *                 propagation.jl:48-53)
+ *   FH_ENOMEM     device allocation failed
+ *   FH_EDEVICE    HIP runtime error
+ *   FH_EUNSUPPORTED a size or mode count this build has no kernel for */
+enum {
+    FH_OK = 0,
+    FH_EINVAL = 1,
+    FH_EGRID = 2,
+    FH_ENOTSYM = 3,
+    FH_ENONFINITE = 4,
+    FH_ENOMEM = 5,
+    FH_EDEVICE = 6,
+    FH_EUNSUPPORTED = 7
+};
+
+/* Arithmetic of the bulk field work.  FH_PREC_FP32_BULK keeps fields, FFTs
+ * and contractions in FP32 and computes the dispersion phase D*z (argument
+ * reduction) and all norms/energies/gradient dot products in FP64. */
+enum { FH_PREC_FP64 = 0, FH_PREC_FP32_BULK = 1 };
+
+/* Integrator.  FH_SCHEME_RK4IP_FIXED is the global-frame interaction-picture
+ * RK4 of _scalar_rk4_output (scalar_crosscheck.jl:53-60), z_i = i*h. */
+enum { FH_SCHEME_RK4IP_FIXED = 0 };
+
+typedef struct fh_context_s *fh_context;
+typedef struct fh_plan_s *fh_plan;
+
+/* Resolved problem: what solve_disp_mmf reads from its fiber/sim dicts. */
+typedef struct {
+    int32_t nt;          /* samples, power of two                           */
+    int32_t modes;       /* M                                               */
+    int32_t precision;   /* FH_PREC_*                                       */
+    int32_t scheme;      /* FH_SCHEME_*                                     */
+    int32_t steps;       /* fixed-step count                                */
+    int32_t max_batch;   /* largest batch passed to one call                */
+    double length_m;     /* fiber["L"]                                      */
+    const double *dispersion; /* fiber["Dw"], nt*modes, rad/m, FFT order    */
+    const double *gamma;      /* fiber["gamma"], modes^4, column-major      */
+    const double *raman;      /* fiber["hRw"], nt complex (2*nt doubles)    */
+    const double *shock;      /* fftshift(sim["ws"]/sim["w0"]), nt          */
+    double one_m_fr;          /* fiber["one_m_fR"]                          */
+} fh_problem_desc;
+
+typedef struct {
+    int64_t accepted;   /* accepted steps (sol.destats.naccept)            */
+    int64_t rejected;   /* rejected steps                                  */
+    int64_t rhs_evals;  /* right-hand-side evaluations (sol.destats.nf)    */
+    int32_t retcode;    /* FH_OK or the failure code                       */
+    int32_t reserved;
+} fh_stats;
+
+/* A context binds one HIP device and one stream (default priority). */
+int fh_context_create(int device, fh_context *out);
+int fh_context_destroy(fh_context ctx);
+
+/* Validates the problem, copies the operators to device memory in the plan's
+ * precision and sizes the scratch for max_batch solves.  The caller keeps
+ * ownership of every array in the descriptor. */
+int fh_plan_create(fh_context ctx, const fh_problem_desc *desc, fh_plan *out);
+int fh_plan_destroy(fh_plan plan);
+
+/* Device bytes the plan holds (operators plus scratch for max_batch). */
+int fh_plan_device_bytes(fh_plan plan, int64_t *bytes);
+
+/* Forward solve of `batch` independent launches sharing the plan's fiber.
+ * u0: batch*modes*nt complex128 launch spectra (interaction picture at z=0,
+ * i.e. problem.uw0); uL: the lab-frame output spectra cis(D L) .* x(L), the
+ * output of _scalar_rk4_output / _lab_spectrum (propagation.jl:34-36).
+ * Host pointers; blocks until done. */
+int fh_forward(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_stats *stats);
+
+/* Same solve on device-resident arrays in the plan's precision (complex128
+ * for FH_PREC_FP64, complex64 for FH_PREC_FP32_BULK), enqueued on `stream`
+ * (a hipStream_t; NULL = the context stream).  Does not synchronise. */
+int fh_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *stream);
+
+/* FP64 per-mode spectral energy E_m = sum_w |u(w,m)|^2 of `batch` device
+ * fields in the plan's precision, written to host energies[batch*modes]
+ * (the mode_energy of _field_metrics, propagation.jl:189-202, up to the
+ * Parseval factor nt*dt). */
+int fh_energy_device(fh_plan plan, int32_t batch, const void *d_u, double *energies);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *fh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIBERHIP_H */

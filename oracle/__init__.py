@@ -1,0 +1,191 @@
+"""The oracle: FP64 CPU restatements of FiberLab's GMMNLSE hot path.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()``
+and ``bench.py``'s CPU-baseline leg, where it is the checker (or the timed
+CPU baseline), never the thing measured or shipped.  The product package
+(libfiberhip and its Python host mirror) never imports it.
+
+Two independent restatements live here:
+
+* ``fiberoracle.c`` (loaded below through ctypes): forward RHS, fixed-step
+  global-frame RK4 (the config-0 oracle), its discrete adjoint and the
+  reference's continuous adjoint RHS, with a self-written radix-2 FFT.
+* ``np_ref.py``: numpy restatement of the setup helpers and the same
+  RHS / RK4 / VJP, using numpy's FFT.
+
+Pins (tests/test_oracle_pins.py): the reference's closed-form tests and its
+published counterfactual numbers; see the test docstrings for file:line.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+from . import np_ref  # noqa: F401  (re-exported)
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB = None
+
+
+class _Fiber(ctypes.Structure):
+    _fields_ = [
+        ("nt", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("disp", ctypes.POINTER(ctypes.c_double)),
+        ("gamma", ctypes.POINTER(ctypes.c_double)),
+        ("hr", ctypes.POINTER(ctypes.c_double)),
+        ("shock", ctypes.POINTER(ctypes.c_double)),
+        ("one_m_fr", ctypes.c_double),
+    ]
+
+
+def build() -> pathlib.Path:
+    """Compile liboracle.so in place with the committed Makefile."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-B", "-C", str(_HERE), "liboracle.so"], check=True)
+    return _HERE / "liboracle.so"
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            build()
+        _LIB = ctypes.CDLL(str(path))
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(_Fiber)
+        _LIB.orc_fft.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        _LIB.orc_rhs.argtypes = [fp, ctypes.c_double, dp, dp]
+        _LIB.orc_rhs_vjp.argtypes = [fp, ctypes.c_double, dp, dp, dp]
+        _LIB.orc_rk4_forward.argtypes = [fp, ctypes.c_double, ctypes.c_int, dp, dp, dp]
+        _LIB.orc_rk4_adjoint.argtypes = [fp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        _LIB.orc_adjoint_rhs.argtypes = [fp, ctypes.c_double, dp, dp, dp]
+        for name in ("orc_fft", "orc_rhs", "orc_rhs_vjp", "orc_rk4_forward", "orc_rk4_adjoint",
+                     "orc_adjoint_rhs"):
+            getattr(_LIB, name).restype = ctypes.c_int
+    return _LIB
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _field(a) -> np.ndarray:
+    """(nt, m) complex -> C-contiguous (m, nt) complex128, i.e. the column-major
+    (t fastest) layout of the (nt, m) array."""
+    a = np.asarray(a, dtype=np.complex128)
+    if a.ndim == 1:
+        a = a[:, None]
+    return np.ascontiguousarray(a.T)
+
+
+class Fiber:
+    """Holds the resolved operators of one problem in the C layout."""
+
+    def __init__(self, D, gamma, hR, shock, one_m_fR):
+        D = np.asarray(D, dtype=np.float64)
+        if D.ndim == 1:
+            D = D[:, None]
+        self.nt, self.m = D.shape
+        self.D = D
+        self._Dc = np.ascontiguousarray(D.T)
+        self.gamma = np.asarray(gamma, dtype=np.float64).reshape((self.m,) * 4)
+        self._gc = np.ascontiguousarray(self.gamma.transpose(3, 2, 1, 0))  # column-major gamma[i,j,k,l]
+        self.hR = np.ascontiguousarray(np.asarray(hR, dtype=np.complex128).reshape(-1))
+        self.shock = np.ascontiguousarray(np.asarray(shock, dtype=np.float64).reshape(-1))
+        self.one_m_fR = float(one_m_fR)
+        self._s = _Fiber(self.nt, self.m, _dp(self._Dc), _dp(self._gc), _dp(self.hR.view(np.float64)),
+                         _dp(self.shock), self.one_m_fR)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self._s)
+
+
+def _check(rc: int, what: str):
+    if rc == -2:
+        raise FloatingPointError(f"{what}: non-finite result")
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (rc={rc})")
+
+
+def fft(x, inverse=False) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+    _check(lib().orc_fft(_dp(a.view(np.float64)), a.size, int(inverse)), "orc_fft")
+    return a
+
+
+def rhs(fib: Fiber, z: float, x) -> np.ndarray:
+    x = _field(x)
+    out = np.zeros_like(x)
+    _check(lib().orc_rhs(fib.ref, z, _dp(x.view(np.float64)), _dp(out.view(np.float64))), "orc_rhs")
+    return out.T
+
+
+def rhs_vjp(fib: Fiber, z: float, x, gy) -> np.ndarray:
+    x, gy = _field(x), _field(gy)
+    out = np.zeros_like(x)
+    _check(lib().orc_rhs_vjp(fib.ref, z, _dp(x.view(np.float64)), _dp(gy.view(np.float64)),
+                             _dp(out.view(np.float64))), "orc_rhs_vjp")
+    return out.T
+
+
+def rk4_forward(fib: Fiber, length: float, steps: int, u0, trajectory=False):
+    """Fixed-step global-frame RK4 (scalar_crosscheck.jl:8-68, any M) -> lab spectrum."""
+    u0 = _field(u0)
+    out = np.zeros_like(u0)
+    traj = np.zeros((steps, fib.m, fib.nt), dtype=np.complex128) if trajectory else None
+    tp = _dp(traj.view(np.float64)) if trajectory else None
+    _check(lib().orc_rk4_forward(fib.ref, length, steps, _dp(u0.view(np.float64)),
+                                 _dp(out.view(np.float64)), tp), "orc_rk4_forward")
+    if trajectory:
+        return out.T, np.transpose(traj, (2, 1, 0))
+    return out.T
+
+
+def rk4_adjoint(fib: Fiber, length: float, steps: int, u0, lam_out, every: int = 50) -> np.ndarray:
+    """Discrete adjoint of rk4_forward: Wirtinger seed at the lab output -> seed at u0."""
+    u0, lo = _field(u0), _field(lam_out)
+    out = np.zeros_like(u0)
+    _check(lib().orc_rk4_adjoint(fib.ref, length, steps, every, _dp(u0.view(np.float64)),
+                                 _dp(lo.view(np.float64)), _dp(out.view(np.float64))), "orc_rk4_adjoint")
+    return out.T
+
+
+def adjoint_rhs(fib: Fiber, z: float, lam, ufwd) -> np.ndarray:
+    """adjoint_disp_mmf! restated (sensitivity_disp_mmf.jl:32-79)."""
+    lam, ufwd = _field(lam), _field(ufwd)
+    out = np.zeros_like(lam)
+    _check(lib().orc_adjoint_rhs(fib.ref, z, _dp(lam.view(np.float64)), _dp(ufwd.view(np.float64)),
+                                 _dp(out.view(np.float64))), "orc_adjoint_rhs")
+    return out.T
+
+
+def continuous_adjoint(fib: Fiber, length: float, steps: int, u0, lam_out) -> np.ndarray:
+    """Backward RK4 of the reference's continuous adjoint ODE (solve_adjoint_disp_mmf,
+    sensitivity_disp_mmf.jl:309-332) with the forward state taken from a forward
+    RK4 run at half the step (so every stage position is a stored state)."""
+    u0 = np.asarray(u0, dtype=np.complex128).reshape(fib.nt, fib.m)
+    _, traj = rk4_forward(fib, length, 2 * steps, u0, trajectory=True)
+    states = [u0] + [traj[:, :, i] for i in range(2 * steps)]
+    h = length / steps
+    lam = np.exp(-1j * fib.D * length) * np.asarray(lam_out, dtype=np.complex128).reshape(fib.nt, fib.m)
+    for n in range(steps, 0, -1):
+        z = n * h
+        f_hi, f_mid, f_lo = states[2 * n], states[2 * n - 1], states[2 * n - 2]
+        k1 = adjoint_rhs(fib, z, lam, f_hi)
+        k2 = adjoint_rhs(fib, z - h / 2, lam - h / 2 * k1, f_mid)
+        k3 = adjoint_rhs(fib, z - h / 2, lam - h / 2 * k2, f_mid)
+        k4 = adjoint_rhs(fib, z - h, lam - h * k3, f_lo)
+        lam = lam - h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return lam
+
+
+__all__ = ["Fiber", "build", "lib", "fft", "rhs", "rhs_vjp", "rk4_forward", "rk4_adjoint",
+           "adjoint_rhs", "continuous_adjoint", "np_ref"]
+_ = os  # keep import for callers that patch paths

@@ -1,0 +1,666 @@
+/*
+ * fiberoracle.c -- CPU FP64 restatement of FiberLab's GMMNLSE propagation
+ * hot path.  TEST INFRASTRUCTURE ONLY: this file is the parity checker for
+ * libfiberhip.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  The reference
+ * itself is Julia (FFTW, Tullio, OrdinaryDiffEq); Julia is not installed in
+ * this image, so the arithmetic below is a line-by-line restatement with a
+ * self-written radix-2 FFT in place of FFTW 3.3.11.
+ *
+ * Layout: Julia column-major, time/frequency index fastest.  A field of
+ * nt samples and m modes is an array of m*nt complex doubles, sample t of
+ * mode k at [k*nt + t].  gamma[i,j,k,l] (1-based in Julia) is stored at
+ * i + m*(j + m*(k + m*l)) with 0-based indices.
+ *
+ * Conventions (Appendix A of the survey, restated from the source):
+ *   fft  = unnormalised, kernel exp(-2 pi i k n / N)      (FFTW.fft)
+ *   ifft = exp(+2 pi i k n / N) scaled by 1/N              (FFTW.ifft)
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------ FFT */
+
+typedef struct {
+    int n, logn;
+    int *rev;
+    cplx *tw; /* exp(-2 pi i k / n), k < n/2, from long-double sin/cos */
+} orc_fft_plan;
+
+static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+
+static int fft_plan_init(orc_fft_plan *p, int n)
+{
+    p->n = n;
+    p->logn = ilog2(n);
+    if ((1 << p->logn) != n || n < 2) return -1;
+    p->rev = (int *)malloc(sizeof(int) * (size_t)n);
+    p->tw = (cplx *)malloc(sizeof(cplx) * (size_t)(n / 2 > 0 ? n / 2 : 1));
+    if (!p->rev || !p->tw) return -1;
+    for (int i = 0; i < n; ++i) {
+        int r = 0;
+        for (int b = 0; b < p->logn; ++b) r |= ((i >> b) & 1) << (p->logn - 1 - b);
+        p->rev[i] = r;
+    }
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n / 2; ++k) {
+        long double a = two_pi * (long double)k / (long double)n;
+        p->tw[k] = (double)cosl(a) - I * (double)sinl(a);
+    }
+    return 0;
+}
+
+static void fft_plan_free(orc_fft_plan *p)
+{
+    free(p->rev);
+    free(p->tw);
+    p->rev = NULL;
+    p->tw = NULL;
+}
+
+/* In-place radix-2 decimation-in-time transform.  inverse != 0 gives
+ * FFTW.ifft semantics (conjugate kernel, 1/N scaling). */
+static void fft_exec(const orc_fft_plan *p, cplx *x, int inverse)
+{
+    const int n = p->n;
+    for (int i = 0; i < n; ++i) {
+        int j = p->rev[i];
+        if (i < j) { cplx t = x[i]; x[i] = x[j]; x[j] = t; }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1, stride = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int k = 0; k < half; ++k) {
+                cplx w = p->tw[k * stride];
+                if (inverse) w = conj(w);
+                cplx a = x[i + k], b = x[i + k + half] * w;
+                x[i + k] = a + b;
+                x[i + k + half] = a - b;
+            }
+        }
+    }
+    if (inverse) {
+        const double s = 1.0 / (double)n;
+        for (int i = 0; i < n; ++i) x[i] *= s;
+    }
+}
+
+/* Exported for convention tests: x has 2*n doubles (interleaved). */
+int orc_fft(double *x, int n, int inverse)
+{
+    orc_fft_plan p;
+    if (fft_plan_init(&p, n)) { fft_plan_free(&p); return -1; }
+    fft_exec(&p, (cplx *)x, inverse);
+    fft_plan_free(&p);
+    return 0;
+}
+
+/* ------------------------------------------------------------ the fiber */
+
+typedef struct {
+    int nt, m;
+    const double *disp;  /* D(w) [m*nt] rad/m, FFT order            */
+    const double *gamma; /* [m^4] column-major gamma[i,j,k,l]        */
+    const double *hr;    /* hR(w) interleaved complex [2*nt]          */
+    const double *shock; /* fftshift(ws/w0) [nt]                      */
+    double one_m_fr;
+} orc_fiber;
+
+#define G4(f, i, j, k, l) ((f)->gamma[(i) + (f)->m * ((j) + (f)->m * ((k) + (f)->m * (l)))])
+
+typedef struct {
+    orc_fft_plan plan;
+    int nt, m;
+    cplx *expp, *expm, *uw, *eta, *dkc, *gu, *gw, *tmp;
+    double *v, *w, *dk, *dr, *h, *a, *b, *c;
+} orc_ws;
+
+static void ws_free(orc_ws *ws)
+{
+    fft_plan_free(&ws->plan);
+    free(ws->expp); free(ws->expm); free(ws->uw); free(ws->eta); free(ws->dkc);
+    free(ws->gu); free(ws->gw); free(ws->tmp);
+    free(ws->v); free(ws->w); free(ws->dk); free(ws->dr); free(ws->h);
+    free(ws->a); free(ws->b); free(ws->c);
+    memset(ws, 0, sizeof(*ws));
+}
+
+static int ws_init(orc_ws *ws, int nt, int m)
+{
+    memset(ws, 0, sizeof(*ws));
+    if (fft_plan_init(&ws->plan, nt)) { ws_free(ws); return -1; }
+    ws->nt = nt;
+    ws->m = m;
+    size_t f = (size_t)nt * (size_t)m, ff = (size_t)nt * (size_t)m * (size_t)m;
+    ws->expp = malloc(sizeof(cplx) * f);
+    ws->expm = malloc(sizeof(cplx) * f);
+    ws->uw = malloc(sizeof(cplx) * f);
+    ws->eta = malloc(sizeof(cplx) * f);
+    ws->gu = malloc(sizeof(cplx) * f);
+    ws->gw = malloc(sizeof(cplx) * f);
+    ws->tmp = malloc(sizeof(cplx) * f);
+    ws->dkc = malloc(sizeof(cplx) * ff);
+    ws->v = malloc(sizeof(double) * f);
+    ws->w = malloc(sizeof(double) * f);
+    ws->dk = malloc(sizeof(double) * ff);
+    ws->dr = malloc(sizeof(double) * ff);
+    ws->h = malloc(sizeof(double) * ff);
+    ws->a = malloc(sizeof(double) * ff);
+    ws->b = malloc(sizeof(double) * ff);
+    ws->c = malloc(sizeof(double) * (size_t)nt * (size_t)m * (size_t)m);
+    if (!ws->expp || !ws->expm || !ws->uw || !ws->eta || !ws->gu || !ws->gw || !ws->tmp ||
+        !ws->dkc || !ws->v || !ws->w || !ws->dk || !ws->dr || !ws->h || !ws->a || !ws->b || !ws->c) {
+        ws_free(ws);
+        return -1;
+    }
+    return 0;
+}
+
+static inline cplx cis(double x) { return cos(x) + I * sin(x); }
+
+/* fftshift along t for even nt: out[t] = in[(t + nt/2) mod nt]. */
+static void shift_real_part(const cplx *in, double *out, int nt)
+{
+    const int h = nt / 2;
+    for (int t = 0; t < nt; ++t) out[t] = creal(in[(t + h) % nt]);
+}
+
+/* Raman filter of one real column, literal sequence of
+ * simulate_disp_mmf.jl:45-50: real(fftshift(ifft(hR .* fft(complex(q))))).
+ * conj_filter != 0 uses conj(hR) (transposed operator, see below). */
+static void raman_apply(orc_ws *ws, const cplx *hr, const double *q, double *out, int conj_filter,
+                        int shift_before)
+{
+    const int nt = ws->nt;
+    cplx *x = ws->tmp;
+    if (shift_before) {
+        for (int t = 0; t < nt; ++t) x[t] = q[(t + nt / 2) % nt];
+    } else {
+        for (int t = 0; t < nt; ++t) x[t] = q[t];
+    }
+    fft_exec(&ws->plan, x, 0);
+    for (int k = 0; k < nt; ++k) x[k] *= conj_filter ? conj(hr[k]) : hr[k];
+    fft_exec(&ws->plan, x, 1);
+    if (shift_before) {
+        for (int t = 0; t < nt; ++t) out[t] = creal(x[t]);
+    } else {
+        shift_real_part(x, out, nt);
+    }
+}
+
+/* Forward RHS, restating disp_mmf! (simulate_disp_mmf.jl:25-60):
+ *   u = fft(cis(Dz) .* x)                                   :28-35
+ *   dK[t,i,j] = sum_kl gamma[i,j,k,l](v_k v_l + w_k w_l)     :38
+ *   etaK = (1-fR)(dK u)                                     :39-42
+ *   dR = real(fftshift(ifft(hR .* fft(dK))))                :45-50
+ *   etaR = dR u                                             :51-53
+ *   dx = i cis(-Dz) .* shock .* ifft(etaK + etaR)           :56-59  */
+static void rhs_eval(const orc_fiber *f, orc_ws *ws, double z, const cplx *x, cplx *dx)
+{
+    const int nt = f->nt, m = f->m;
+    const cplx *hr = (const cplx *)f->hr;
+    for (int k = 0; k < m; ++k)
+        for (int t = 0; t < nt; ++t) {
+            size_t o = (size_t)k * nt + t;
+            ws->expp[o] = cis(f->disp[o] * z);
+            ws->expm[o] = cis(-f->disp[o] * z);
+            ws->uw[o] = ws->expp[o] * x[o];
+        }
+    for (int k = 0; k < m; ++k) fft_exec(&ws->plan, ws->uw + (size_t)k * nt, 0);
+    for (size_t o = 0; o < (size_t)nt * m; ++o) {
+        ws->v[o] = creal(ws->uw[o]);
+        ws->w[o] = cimag(ws->uw[o]);
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double *col = ws->dk + ((size_t)i + (size_t)m * j) * nt;
+            for (int t = 0; t < nt; ++t) {
+                double s = 0.0;
+                for (int k = 0; k < m; ++k)
+                    for (int l = 0; l < m; ++l)
+                        s += G4(f, i, j, k, l) *
+                             (ws->v[(size_t)k * nt + t] * ws->v[(size_t)l * nt + t] +
+                              ws->w[(size_t)k * nt + t] * ws->w[(size_t)l * nt + t]);
+                col[t] = s;
+            }
+        }
+    for (int p = 0; p < m * m; ++p)
+        raman_apply(ws, hr, ws->dk + (size_t)p * nt, ws->dr + (size_t)p * nt, 0, 0);
+    for (int i = 0; i < m; ++i)
+        for (int t = 0; t < nt; ++t) {
+            double ak = 0, bk = 0, ar = 0, br = 0;
+            for (int j = 0; j < m; ++j) {
+                double d = ws->dk[((size_t)i + (size_t)m * j) * nt + t];
+                double r = ws->dr[((size_t)i + (size_t)m * j) * nt + t];
+                double vj = ws->v[(size_t)j * nt + t], wj = ws->w[(size_t)j * nt + t];
+                ak += d * vj;
+                bk += d * wj;
+                ar += r * vj;
+                br += r * wj;
+            }
+            cplx etak = (ak + I * bk) * f->one_m_fr;
+            cplx etar = ar + I * br;
+            ws->eta[(size_t)i * nt + t] = etak + etar;
+        }
+    for (int k = 0; k < m; ++k) fft_exec(&ws->plan, ws->eta + (size_t)k * nt, 1);
+    for (int k = 0; k < m; ++k)
+        for (int t = 0; t < nt; ++t) {
+            size_t o = (size_t)k * nt + t;
+            dx[o] = I * ws->expm[o] * (ws->eta[o] * f->shock[t]);
+        }
+}
+
+int orc_rhs(const orc_fiber *f, double z, const double *x, double *dx)
+{
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    rhs_eval(f, &ws, z, (const cplx *)x, (cplx *)dx);
+    ws_free(&ws);
+    return 0;
+}
+
+/* ------------------------------------------------- fixed-step RK4 (IP) */
+
+/* One classic RK4 step in the global frame, restating the loop body of
+ * _scalar_rk4_output (scalar_crosscheck.jl:53-60), generalised to M modes
+ * with the disp_mmf! right-hand side.  z = index*h, stages at z, z+h/2,
+ * z+h/2, z+h.  Optional stage outputs y2..y4 (stage inputs) for the adjoint. */
+static void rk4_step(const orc_fiber *f, orc_ws *ws, int index, double h, cplx *x, cplx *k1, cplx *k2,
+                     cplx *k3, cplx *k4, cplx *y)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    const double z = index * h;
+    rhs_eval(f, ws, z, x, k1);
+    for (size_t o = 0; o < nf; ++o) y[o] = x[o] + h / 2 * k1[o];
+    rhs_eval(f, ws, z + h / 2, y, k2);
+    for (size_t o = 0; o < nf; ++o) y[o] = x[o] + h / 2 * k2[o];
+    rhs_eval(f, ws, z + h / 2, y, k3);
+    for (size_t o = 0; o < nf; ++o) y[o] = x[o] + h * k3[o];
+    rhs_eval(f, ws, z + h, y, k4);
+    for (size_t o = 0; o < nf; ++o) x[o] += h / 6 * (k1[o] + 2 * k2[o] + 2 * k3[o] + k4[o]);
+}
+
+/* _scalar_rk4_output (scalar_crosscheck.jl:8-68) for M >= 1.  u0 is the
+ * interaction-picture launch (= uw0), out is cis(D L) .* x(L) (:61-65).
+ * If traj != NULL it receives x after every step (steps*nt*m values). */
+int orc_rk4_forward(const orc_fiber *f, double length_m, int steps, const double *u0, double *out,
+                    double *traj)
+{
+    if (steps <= 0) return -1;
+    const size_t nf = (size_t)f->nt * f->m;
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    cplx *x = malloc(sizeof(cplx) * nf), *k = malloc(sizeof(cplx) * nf * 5);
+    if (!x || !k) { free(x); free(k); ws_free(&ws); return -1; }
+    memcpy(x, u0, sizeof(cplx) * nf);
+    const double h = length_m / steps;
+    for (int s = 0; s < steps; ++s) {
+        rk4_step(f, &ws, s, h, x, k, k + nf, k + 2 * nf, k + 3 * nf, k + 4 * nf);
+        if (traj) memcpy(traj + 2 * nf * (size_t)s, x, sizeof(cplx) * nf);
+    }
+    cplx *o = (cplx *)out;
+    int finite = 1;
+    for (size_t i = 0; i < nf; ++i) {
+        o[i] = cis(f->disp[i] * length_m) * x[i];
+        if (!isfinite(creal(o[i])) || !isfinite(cimag(o[i]))) finite = 0;
+    }
+    free(x);
+    free(k);
+    ws_free(&ws);
+    return finite ? 0 : -2;
+}
+
+/* ------------------------------------------ discrete adjoint (our VJP) */
+
+/* Vector-Jacobian product of the RHS above.  The RHS is real-linear in x,
+ * so for a real objective J with cotangent g = dJ/dRe + i dJ/dIm (or the
+ * Wirtinger seed dJ/dconj, which differs by the exact factor 2) the
+ * pullback is
+ *   G_i     = fft(-i cis(D_i z) S gy_i) / N
+ *   gu_j    = sum_i h_ij G_i,        h_ij = (1-fR) dK_ij + dR_ij
+ *   A_ij    = Re(conj(G_i) u_j)
+ *   B_ij    = (1-fR) A_ij + R^T(A_ij),  R^T(a) = real(ifft(conj(hR) fft(shift(a))))
+ *   C_kl    = sum_ij B_ij gamma[i,j,k,l]
+ *   gu_k   += sum_l (C_kl + C_lk) u_l
+ *   gx_m    = conj(cis(D_m z)) N ifft(gu_m)
+ * The continuous counterpart is adjoint_disp_mmf! (sensitivity_disp_mmf.jl:32-79),
+ * which uses the same conj(hR) and sign-alternation (sigma1, :241-242). */
+static void rhs_vjp(const orc_fiber *f, orc_ws *ws, double z, const cplx *x, const cplx *gy, cplx *gx)
+{
+    const int nt = f->nt, m = f->m;
+    const size_t nf = (size_t)nt * m;
+    const cplx *hr = (const cplx *)f->hr;
+    for (size_t o = 0; o < nf; ++o) {
+        ws->expp[o] = cis(f->disp[o] * z);
+        ws->uw[o] = ws->expp[o] * x[o];
+    }
+    for (int k = 0; k < m; ++k) fft_exec(&ws->plan, ws->uw + (size_t)k * nt, 0);
+    for (size_t o = 0; o < nf; ++o) {
+        ws->v[o] = creal(ws->uw[o]);
+        ws->w[o] = cimag(ws->uw[o]);
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double *col = ws->dk + ((size_t)i + (size_t)m * j) * nt;
+            for (int t = 0; t < nt; ++t) {
+                double s = 0.0;
+                for (int k = 0; k < m; ++k)
+                    for (int l = 0; l < m; ++l)
+                        s += G4(f, i, j, k, l) *
+                             (ws->v[(size_t)k * nt + t] * ws->v[(size_t)l * nt + t] +
+                              ws->w[(size_t)k * nt + t] * ws->w[(size_t)l * nt + t]);
+                col[t] = s;
+            }
+        }
+    for (int p = 0; p < m * m; ++p) {
+        raman_apply(ws, hr, ws->dk + (size_t)p * nt, ws->dr + (size_t)p * nt, 0, 0);
+        for (int t = 0; t < nt; ++t)
+            ws->h[(size_t)p * nt + t] = f->one_m_fr * ws->dk[(size_t)p * nt + t] + ws->dr[(size_t)p * nt + t];
+    }
+    /* G = fft(-i P S gy)/N */
+    for (int k = 0; k < m; ++k)
+        for (int t = 0; t < nt; ++t) {
+            size_t o = (size_t)k * nt + t;
+            ws->gw[o] = -I * ws->expp[o] * (f->shock[t] * gy[o]);
+        }
+    for (int k = 0; k < m; ++k) {
+        fft_exec(&ws->plan, ws->gw + (size_t)k * nt, 0);
+        for (int t = 0; t < nt; ++t) ws->gw[(size_t)k * nt + t] /= (double)nt;
+    }
+    for (int j = 0; j < m; ++j)
+        for (int t = 0; t < nt; ++t) {
+            cplx s = 0;
+            for (int i = 0; i < m; ++i)
+                s += ws->h[((size_t)i + (size_t)m * j) * nt + t] * ws->gw[(size_t)i * nt + t];
+            ws->gu[(size_t)j * nt + t] = s;
+        }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            for (int t = 0; t < nt; ++t)
+                ws->a[((size_t)i + (size_t)m * j) * nt + t] =
+                    creal(conj(ws->gw[(size_t)i * nt + t]) * ws->uw[(size_t)j * nt + t]);
+    for (int p = 0; p < m * m; ++p) {
+        raman_apply(ws, hr, ws->a + (size_t)p * nt, ws->b + (size_t)p * nt, 1, 1);
+        for (int t = 0; t < nt; ++t)
+            ws->b[(size_t)p * nt + t] += f->one_m_fr * ws->a[(size_t)p * nt + t];
+    }
+    for (int k = 0; k < m; ++k)
+        for (int l = 0; l < m; ++l)
+            for (int t = 0; t < nt; ++t) {
+                double s = 0;
+                for (int i = 0; i < m; ++i)
+                    for (int j = 0; j < m; ++j)
+                        s += ws->b[((size_t)i + (size_t)m * j) * nt + t] * G4(f, i, j, k, l);
+                ws->c[((size_t)k + (size_t)m * l) * nt + t] = s;
+            }
+    for (int k = 0; k < m; ++k)
+        for (int t = 0; t < nt; ++t) {
+            cplx s = 0;
+            for (int l = 0; l < m; ++l)
+                s += (ws->c[((size_t)k + (size_t)m * l) * nt + t] + ws->c[((size_t)l + (size_t)m * k) * nt + t]) *
+                     ws->uw[(size_t)l * nt + t];
+            ws->gu[(size_t)k * nt + t] += s;
+        }
+    for (int k = 0; k < m; ++k) {
+        cplx *col = ws->gu + (size_t)k * nt;
+        fft_exec(&ws->plan, col, 1);
+        for (int t = 0; t < nt; ++t) {
+            size_t o = (size_t)k * nt + t;
+            gx[o] = conj(ws->expp[o]) * (col[t] * (double)nt);
+        }
+    }
+}
+
+int orc_rhs_vjp(const orc_fiber *f, double z, const double *x, const double *gy, double *gx)
+{
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    rhs_vjp(f, &ws, z, (const cplx *)x, (const cplx *)gy, (cplx *)gx);
+    ws_free(&ws);
+    return 0;
+}
+
+/* Pull a cotangent lam (of x_{n+1}) back through one RK4 step from x_n. */
+static void rk4_step_vjp(const orc_fiber *f, orc_ws *ws, int index, double h, const cplx *x, cplx *lam,
+                         cplx *k1, cplx *k2, cplx *k3, cplx *y2, cplx *y3, cplx *y4, cplx *g, cplx *gk)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    const double z = index * h;
+    /* recompute stage inputs */
+    rhs_eval(f, ws, z, x, k1);
+    for (size_t o = 0; o < nf; ++o) y2[o] = x[o] + h / 2 * k1[o];
+    rhs_eval(f, ws, z + h / 2, y2, k2);
+    for (size_t o = 0; o < nf; ++o) y3[o] = x[o] + h / 2 * k2[o];
+    rhs_eval(f, ws, z + h / 2, y3, k3);
+    for (size_t o = 0; o < nf; ++o) y4[o] = x[o] + h * k3[o];
+    /* cotangents of k_s: x_{n+1} = x + h/6 (k1 + 2k2 + 2k3 + k4) */
+    cplx *acc = k1; /* reuse k1 storage for the running cotangent of x */
+    for (size_t o = 0; o < nf; ++o) acc[o] = lam[o];
+    /* stage 4: gk4 = h/6 lam */
+    for (size_t o = 0; o < nf; ++o) gk[o] = h / 6 * lam[o];
+    rhs_vjp(f, ws, z + h, y4, gk, g);
+    for (size_t o = 0; o < nf; ++o) { acc[o] += g[o]; k2[o] = h * g[o]; } /* k2 := extra cotangent on k3 */
+    /* stage 3: gk3 = h/3 lam + h g4 */
+    for (size_t o = 0; o < nf; ++o) gk[o] = h / 3 * lam[o] + k2[o];
+    rhs_vjp(f, ws, z + h / 2, y3, gk, g);
+    for (size_t o = 0; o < nf; ++o) { acc[o] += g[o]; k2[o] = h / 2 * g[o]; }
+    /* stage 2: gk2 = h/3 lam + h/2 g3 */
+    for (size_t o = 0; o < nf; ++o) gk[o] = h / 3 * lam[o] + k2[o];
+    rhs_vjp(f, ws, z + h / 2, y2, gk, g);
+    for (size_t o = 0; o < nf; ++o) { acc[o] += g[o]; k2[o] = h / 2 * g[o]; }
+    /* stage 1: gk1 = h/6 lam + h/2 g2 */
+    for (size_t o = 0; o < nf; ++o) gk[o] = h / 6 * lam[o] + k2[o];
+    rhs_vjp(f, ws, z, x, gk, g);
+    for (size_t o = 0; o < nf; ++o) lam[o] = acc[o] + g[o];
+    (void)k3;
+}
+
+/* Discrete adjoint of orc_rk4_forward.  lam_out is the Wirtinger seed
+ * dJ/dconj(u_lab(L)) (terminal_adjoint convention, adjoints.jl:723-739);
+ * lam0 receives dJ/dconj(u0).  The terminal rotation exp(-i D L) restates
+ * solve_adjoint_disp_mmf (sensitivity_disp_mmf.jl:311).  Checkpoints every
+ * `every` steps; each segment is recomputed once. */
+int orc_rk4_adjoint(const orc_fiber *f, double length_m, int steps, int every, const double *u0,
+                    const double *lam_out, double *lam0)
+{
+    if (steps <= 0 || every <= 0) return -1;
+    const size_t nf = (size_t)f->nt * f->m;
+    const int nseg = (steps + every - 1) / every;
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    cplx *ck = malloc(sizeof(cplx) * nf * (size_t)nseg);
+    cplx *seg = malloc(sizeof(cplx) * nf * (size_t)every);
+    cplx *k = malloc(sizeof(cplx) * nf * 9);
+    cplx *x = malloc(sizeof(cplx) * nf), *lam = malloc(sizeof(cplx) * nf);
+    if (!ck || !seg || !k || !x || !lam) {
+        free(ck); free(seg); free(k); free(x); free(lam); ws_free(&ws);
+        return -1;
+    }
+    const double h = length_m / steps;
+    memcpy(x, u0, sizeof(cplx) * nf);
+    for (int s = 0; s < steps; ++s) {
+        if (s % every == 0) memcpy(ck + nf * (size_t)(s / every), x, sizeof(cplx) * nf);
+        rk4_step(f, &ws, s, h, x, k, k + nf, k + 2 * nf, k + 3 * nf, k + 4 * nf);
+    }
+    const cplx *lo = (const cplx *)lam_out;
+    for (size_t i = 0; i < nf; ++i) lam[i] = cis(-f->disp[i] * length_m) * lo[i];
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * every, s1 = (s0 + every < steps) ? s0 + every : steps;
+        memcpy(x, ck + nf * (size_t)sg, sizeof(cplx) * nf);
+        for (int s = s0; s < s1; ++s) {
+            memcpy(seg + nf * (size_t)(s - s0), x, sizeof(cplx) * nf);
+            if (s + 1 < s1) rk4_step(f, &ws, s, h, x, k, k + nf, k + 2 * nf, k + 3 * nf, k + 4 * nf);
+        }
+        for (int s = s1 - 1; s >= s0; --s)
+            rk4_step_vjp(f, &ws, s, h, seg + nf * (size_t)(s - s0), lam, k, k + nf, k + 2 * nf, k + 3 * nf,
+                         k + 4 * nf, k + 5 * nf, k + 6 * nf, k + 7 * nf);
+    }
+    memcpy(lam0, lam, sizeof(cplx) * nf);
+    int finite = 1;
+    for (size_t i = 0; i < nf; ++i)
+        if (!isfinite(creal(lam[i])) || !isfinite(cimag(lam[i]))) finite = 0;
+    free(ck); free(seg); free(k); free(x); free(lam);
+    ws_free(&ws);
+    return finite ? 0 : -2;
+}
+
+/* --------------------------------- continuous adjoint RHS (reference) */
+
+/* Restatement of adjoint_disp_mmf! (sensitivity_disp_mmf.jl:32-79) with its
+ * helpers calc_lambda_dfKR1c_duc! (:141-145), calc_lambdac_dfK_duc! (:153-157),
+ * calc_lambda_dfR2c_duc! (:93-101), calc_lambdac_dfR_duc! (:123-132),
+ * calc_gamma_a_b! (:169-184) and calc_deltas! (:199-215).  tau = shock,
+ * sigma1[k] = (-1)^k (:241-242).  ufwd is the forward state x(z). */
+int orc_adjoint_rhs(const orc_fiber *f, double z, const double *lam_in, const double *ufwd, double *dlam_out)
+{
+    const int nt = f->nt, m = f->m;
+    const size_t nf = (size_t)nt * m, ff = nf * (size_t)m;
+    const cplx *lamt = (const cplx *)lam_in, *uf = (const cplx *)ufwd;
+    const cplx *hr = (const cplx *)f->hr;
+    cplx *dl = (cplx *)dlam_out;
+    orc_fft_plan p;
+    if (fft_plan_init(&p, nt)) { fft_plan_free(&p); return -1; }
+    cplx *ep = malloc(sizeof(cplx) * nf), *em = malloc(sizeof(cplx) * nf);
+    cplx *lw = malloc(sizeof(cplx) * nf), *lt = malloc(sizeof(cplx) * nf), *lwc = malloc(sizeof(cplx) * nf);
+    cplx *uw = malloc(sizeof(cplx) * nf), *ut = malloc(sizeof(cplx) * nf), *utc = malloc(sizeof(cplx) * nf);
+    double *d1 = malloc(sizeof(double) * ff), *dr1 = malloc(sizeof(double) * ff), *dkr = malloc(sizeof(double) * ff);
+    cplx *d2 = malloc(sizeof(cplx) * ff), *tmpm = malloc(sizeof(cplx) * ff);
+    cplx *sr = malloc(sizeof(cplx) * nf), *t1 = malloc(sizeof(cplx) * nf), *t2 = malloc(sizeof(cplx) * nf);
+    cplx *t3 = malloc(sizeof(cplx) * nf), *t4 = malloc(sizeof(cplx) * nf), *col = malloc(sizeof(cplx) * nt);
+    int rc = 0;
+    if (!ep || !em || !lw || !lt || !lwc || !uw || !ut || !utc || !d1 || !dr1 || !dkr || !d2 || !tmpm || !sr ||
+        !t1 || !t2 || !t3 || !t4 || !col) {
+        rc = -1;
+        goto done;
+    }
+    for (size_t o = 0; o < nf; ++o) {
+        ep[o] = cis(f->disp[o] * z);
+        em[o] = cis(-f->disp[o] * z);
+    }
+    /* :43-47 */
+    for (int k = 0; k < m; ++k)
+        for (int t = 0; t < nt; ++t) {
+            size_t o = (size_t)k * nt + t;
+            lw[o] = ep[o] * lamt[o] * f->shock[t];
+            lt[o] = lw[o];
+            lwc[o] = conj(lw[o]);
+        }
+    for (int k = 0; k < m; ++k) {
+        fft_exec(&p, lt + (size_t)k * nt, 0);
+        fft_exec(&p, lwc + (size_t)k * nt, 1);
+    }
+    /* :50-54 */
+    for (size_t o = 0; o < nf; ++o) {
+        uw[o] = ep[o] * uf[o];
+        ut[o] = uw[o];
+    }
+    for (int k = 0; k < m; ++k) fft_exec(&p, ut + (size_t)k * nt, 0);
+    for (size_t o = 0; o < nf; ++o) utc[o] = conj(ut[o]);
+    for (int k = 0; k < m; ++k) fft_exec(&p, uw + (size_t)k * nt, 1);
+    /* calc_deltas! :199-215: d1[t,i,j] = sum_kl (v_k v_l + w_k w_l) gamma[l,k,i,j]
+     * d2 = sum_kl (v_k v_l - w_k w_l + 2i v_k w_l) gamma[l,k,i,j] */
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            for (int t = 0; t < nt; ++t) {
+                double s1 = 0, s2r = 0, s2i = 0;
+                for (int k = 0; k < m; ++k)
+                    for (int l = 0; l < m; ++l) {
+                        double vk = creal(ut[(size_t)k * nt + t]), wk = cimag(ut[(size_t)k * nt + t]);
+                        double vl = creal(ut[(size_t)l * nt + t]), wl = cimag(ut[(size_t)l * nt + t]);
+                        double g = G4(f, l, k, i, j);
+                        s1 += (vk * vl + wk * wl) * g;
+                        s2r += (vk * vl - wk * wl) * g;
+                        s2i += (2 * vk * wl) * g;
+                    }
+                size_t o = ((size_t)i + (size_t)m * j) * nt + t;
+                d1[o] = s1;
+                d2[o] = s2r + I * s2i;
+            }
+    /* :60-69 */
+    for (int q = 0; q < m * m; ++q) {
+        for (int t = 0; t < nt; ++t) col[t] = d1[(size_t)q * nt + t];
+        fft_exec(&p, col, 0);
+        for (int t = 0; t < nt; ++t) col[t] *= hr[t];
+        fft_exec(&p, col, 1);
+        for (int t = 0; t < nt; ++t) dr1[(size_t)q * nt + t] = creal(col[(t + nt / 2) % nt]);
+        for (int t = 0; t < nt; ++t)
+            dkr[(size_t)q * nt + t] = 2 * f->one_m_fr * d1[(size_t)q * nt + t] + dr1[(size_t)q * nt + t];
+    }
+    /* term 1 :141-145: sum_res[t,j] = lt[t,i] dkr[t,i,j]; ifft; 1i*em */
+    for (int j = 0; j < m; ++j)
+        for (int t = 0; t < nt; ++t) {
+            cplx s = 0;
+            for (int i = 0; i < m; ++i) s += lt[(size_t)i * nt + t] * dkr[((size_t)i + (size_t)m * j) * nt + t];
+            sr[(size_t)j * nt + t] = s;
+        }
+    for (int k = 0; k < m; ++k) fft_exec(&p, sr + (size_t)k * nt, 1);
+    for (size_t o = 0; o < nf; ++o) t1[o] = I * em[o] * sr[o];
+    /* term 2 :153-157: sum_res[t,j] = lwc[t,i] d2[t,i,j]; ifft; -1i*Nt*em */
+    for (int j = 0; j < m; ++j)
+        for (int t = 0; t < nt; ++t) {
+            cplx s = 0;
+            for (int i = 0; i < m; ++i) s += lwc[(size_t)i * nt + t] * d2[((size_t)i + (size_t)m * j) * nt + t];
+            sr[(size_t)j * nt + t] = s;
+        }
+    for (int k = 0; k < m; ++k) fft_exec(&p, sr + (size_t)k * nt, 1);
+    for (size_t o = 0; o < nf; ++o) t2[o] = -I * (double)nt * em[o] * sr[o];
+    /* term 3 :93-101: gab = gamma(lt, utc); fft; *= conj(hR) sigma; fft; contract with ifft(uw) (uw already
+     * ifft'd at :54); fft; 1i/Nt em */
+    for (int pass = 0; pass < 2; ++pass) {
+        const cplx *a = pass == 0 ? lt : lwc;
+        const cplx *b = pass == 0 ? utc : ut;
+        /* calc_gamma_a_b! :169-184: gab[t,i,j] = sum_lk (a_l b_k) gamma[l,k,i,j] */
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j)
+                for (int t = 0; t < nt; ++t) {
+                    double sre = 0, sim = 0;
+                    for (int l = 0; l < m; ++l)
+                        for (int k = 0; k < m; ++k) {
+                            cplx ab = a[(size_t)l * nt + t] * b[(size_t)k * nt + t];
+                            double g = G4(f, l, k, i, j);
+                            sre += creal(ab) * g;
+                            sim += cimag(ab) * g;
+                        }
+                    tmpm[((size_t)i + (size_t)m * j) * nt + t] = sre + I * sim;
+                }
+        for (int q = 0; q < m * m; ++q) {
+            cplx *c = tmpm + (size_t)q * nt;
+            if (pass == 0) {
+                fft_exec(&p, c, 0);
+                for (int t = 0; t < nt; ++t) c[t] *= conj(hr[t]) * ((t & 1) ? -1.0 : 1.0);
+                fft_exec(&p, c, 0);
+            } else {
+                fft_exec(&p, c, 1);
+                for (int t = 0; t < nt; ++t) c[t] *= hr[t] * ((t & 1) ? -1.0 : 1.0);
+                fft_exec(&p, c, 0);
+            }
+        }
+        const cplx *rhsf = pass == 0 ? uw : ut;
+        for (int i = 0; i < m; ++i)
+            for (int t = 0; t < nt; ++t) {
+                cplx s = 0;
+                for (int j = 0; j < m; ++j) s += tmpm[((size_t)i + (size_t)m * j) * nt + t] * rhsf[(size_t)j * nt + t];
+                sr[(size_t)i * nt + t] = s;
+            }
+        for (int k = 0; k < m; ++k) fft_exec(&p, sr + (size_t)k * nt, pass == 0 ? 0 : 1);
+        if (pass == 0)
+            for (size_t o = 0; o < nf; ++o) t3[o] = I / (double)nt * em[o] * sr[o];
+        else
+            for (size_t o = 0; o < nf; ++o) t4[o] = -I * (double)nt * em[o] * sr[o];
+    }
+    /* :78 */
+    for (size_t o = 0; o < nf; ++o) dl[o] = t1[o] + f->one_m_fr * t2[o] + t3[o] + t4[o];
+done:
+    free(ep); free(em); free(lw); free(lt); free(lwc); free(uw); free(ut); free(utc);
+    free(d1); free(dr1); free(dkr); free(d2); free(tmpm); free(sr); free(t1); free(t2);
+    free(t3); free(t4); free(col);
+    fft_plan_free(&p);
+    return rc;
+}
