@@ -1,0 +1,192 @@
+"""bench.py -- libfiberhip throughput on one MI355X.
+
+A "step" is one launch of the fused RK4IP kernel over a batch of B
+independent solves of the workload (synthetic launches: the workload's pulse
+with seeded Taylor spectral phases, as in the candidate sweep).  Inputs are
+resident in HBM before the timed region; the timed region is K launches
+bracketed by synchronisations, timed with HIP events on the launch stream.
+
+Prints one JSON line (see README / DESIGN for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import threading
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
+
+import numpy as np  # noqa: E402
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense vector/matrix peaks
+
+
+def rhs_flops(nt: int, m: int) -> float:
+    """Algorithmic flops of one disp_mmf! evaluation (FFTs 5 N log2 N each,
+    2M + 2M^2 of them; gamma contraction 2M^4 N; pointwise work)."""
+    return 5 * nt * math.log2(nt) * (2 * m + 2 * m * m) + 2 * m ** 4 * nt + 17 * m * m * nt + 16 * m * nt
+
+
+def forward_flops(nt: int, m: int, steps: int) -> float:
+    """Fixed-step RK4IP solve: 4 RHS per step plus 28 MN for the stage updates."""
+    return steps * (4 * rhs_flops(nt, m) + 28 * m * nt)
+
+
+def workload(name: str):
+    import fiberhip
+
+    if name == "config0":
+        fib = fiberhip.Fiber(length_m=0.5, power_w=0.1, preset="SMF28", beta_order=3)
+        problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=8192, time_window_ps=10.0),
+                                         raman_threshold_thz=None)
+        return problem, 200
+    raise SystemExit(f"unknown workload {name}")
+
+
+def launches(problem, batch: int, seed: int = 3) -> np.ndarray:
+    import fiberhip
+
+    rng = np.random.default_rng(seed)
+    basis = fiberhip.taylor_phase_basis(problem, (2, 3, 4))
+    coef = np.stack([rng.uniform(-1e4, 1e4, batch), rng.uniform(-5e5, 5e5, batch),
+                     rng.uniform(-1e7, 1e7, batch)], axis=1)
+    phases = coef @ basis.T  # (batch, nt)
+    return problem.uω0[None, :, 0] * np.exp(1j * phases)  # (batch, nt)
+
+
+def time_path(problem, steps, precision, batch, k, warmup):
+    import torch
+
+    import fiberhip
+
+    plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch)
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    u0 = torch.from_numpy(launches(problem, batch)).to(dt).cuda()
+    out = torch.empty_like(u0)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        plan.forward_device(batch, u0.data_ptr(), out.data_ptr(), sp)
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+    t0 = time.perf_counter()
+    for a, b in events:
+        a.record(stream)
+        plan.forward_device(batch, u0.data_ptr(), out.data_ptr(), sp)
+        b.record(stream)
+        print(f"bench: {precision} launch of {batch} solves enqueued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [a.elapsed_time(b) for a, b in events]
+    energies = plan.energy_device(batch, out.data_ptr())
+    return dict(plan=plan, out=out, wall_s=wall, kernel_ms=float(np.mean(ms)), kernel_ms_p50=float(np.median(ms)),
+                energies=energies)
+
+
+def cpu_baseline(problem, steps, budget_s=20.0):
+    """The oracle's C restatement (FP64, one thread per solve) on the host
+    cores: all-core throughput of independent solves, as the reference
+    recommends parallelising solves rather than FFTs."""
+    import fiberhip
+    import oracle
+
+    f = problem.fiber
+    fib = oracle.Fiber(f["Dω"], f["γ"], f["hRω"], fiberhip.self_steepening(problem.sim), f["one_m_fR"])
+    u = launches(problem, 64)
+    t = time.perf_counter()
+    oracle.rk4_forward(fib, f["L"], steps, u[0][:, None])
+    single = time.perf_counter() - t
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    per_thread = max(1, int(budget_s / single / 2))
+    done = [0]
+    lock = threading.Lock()
+
+    def work(i):
+        for j in range(per_thread):
+            oracle.rk4_forward(fib, f["L"], steps, u[(i * per_thread + j) % 64][:, None])
+            with lock:
+                done[0] += 1
+
+    t = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t
+    return dict(value=done[0] / wall, unit="solves/s", cores=threads, kind="port",
+                sample=f"{done[0]} config-0 solves ({steps} RK4 steps, Nt={problem.nt}) on {threads} threads, "
+                       f"one solve per thread at a time; single-thread latency {single:.3f} s",
+                single_thread_s=single)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--workload", default="config0")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (independent solves shard across GPUs without "
+              "communication; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        raise SystemExit(2)
+    import torch
+
+    torch.cuda.init()
+    problem, steps = workload(args.workload)
+    nt, m = problem.nt, problem.modes
+    res = {}
+    for precision in ("fp32", "fp64"):
+        res[precision] = time_path(problem, steps, precision, args.batch, args.steps, args.warmup)
+    e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
+    err = float(np.max(np.abs(e32 - e64) / e64))
+    flops = forward_flops(nt, m, steps)
+    head = res["fp32"]
+    ms_step = head["kernel_ms"]
+    value = args.batch / (ms_step / 1e3)
+    fp64_value = args.batch / (res["fp64"]["kernel_ms"] / 1e3)
+    achieved = args.batch * flops / (ms_step / 1e3) / 1e12
+    line = {
+        "metric": "fwd solves/sec (FP32-bulk batched path) vs plain FP64 GPU @<=1e-5 err",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: SMF-28 0.5 m, 0.1 W, 185 fs sech2, Nt={nt}, "
+                               f"T={problem.sim['time_window']} ps, {steps} RK4IP steps",
+                   "batch_solves_per_step": args.batch, "modes": m, "nt": nt},
+        "fp64_value": fp64_value,
+        "fp32_over_fp64": value / fp64_value,
+        "fp32_vs_fp64_max_rel_energy_err": err,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+                     "kernel": "k_rk4ip_single<float,8192,512>",
+                     "flops_per_solve": flops},
+        "fp64_roofline": {"achieved": args.batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12,
+                          "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                          "kernel": "k_rk4ip_single<double,8192,1024>"},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(problem, steps)
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+"""CPU tests of the host side: the FiberLab-surface mirror against the oracle's
+restatement of the same helpers, error behaviour, and the C ABI library
+(loads, exports every symbol include/fiberhip.h declares).  No GPU calls."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import fiberhip
+from fiberhip import _lib
+from oracle import np_ref as R
+
+
+def test_config0_grid_resolution():
+    """Config 0 asks for Nt=2^13 at 10 ps; the reference's auto grid policy
+    resolves it to Nt=8192, T=23.531 ps (physics_helpers.jl:138-185)."""
+    fib = fiberhip.Fiber(length_m=0.5, power_w=0.1)
+    g = fiberhip.resolve_grid(fib, fiberhip.Pulse(), fiberhip.Grid(nt=8192, time_window_ps=10.0))
+    assert g.nt == 8192 and g.time_window_ps == 23.531 and g.policy == "exact"
+    # default grid (test_numerical_kernels.jl:300-303)
+    assert fiberhip.resolve_sampling_grid(fiberhip.Grid()) == fiberhip.Grid(nt=1024, time_window_ps=10.0,
+                                                                             policy="exact")
+    with pytest.raises(ValueError, match="use at least 21.178 ps"):
+        fiberhip.resolve_sampling_grid(fiberhip.Grid(nt=8192, time_window_ps=12.0, policy="exact"))
+
+
+@pytest.mark.parametrize("nt,window", [(256, 3.0), (1024, 10.0), (8192, 23.531)])
+def test_setup_matches_oracle(nt, window):
+    sim = fiberhip.get_disp_sim_params(1550e-9, 1, nt, window, 3)
+    osim = R.sim_params(1550e-9, 1, nt, window, 3)
+    assert np.array_equal(sim["ts"], osim["ts"]) and np.allclose(sim["ωs"], osim["ws"], rtol=1e-15)
+    fib = fiberhip.get_disp_fiber_params_user_defined(0.5, sim, gamma_user=1.1e-3,
+                                                      betas_user=[-2.17e-26, 1.2e-40])
+    assert np.allclose(fib["Dω"], R.dispersion_user(osim, [-2.17e-26, 1.2e-40]), rtol=1e-15, atol=0)
+    assert np.allclose(fib["hRω"], R.raman_fields(osim)["hR"], rtol=1e-15, atol=1e-300)
+    assert np.allclose(fiberhip.self_steepening(sim), R.shock(osim), rtol=1e-15)
+    _, u = fiberhip.get_initial_state([1.0], 0.1, 185e-15, 80.5e6, "sech_sq", sim)
+    _, v = R.initial_state([1.0], 0.1, 185e-15, 80.5e6, "sech_sq", osim)
+    assert np.array_equal(u, v)
+    # sech^2 peak power 0.881374 P/(fwhm rep) (simulate_disp_mmf.jl:150)
+    ut = np.fft.fft(u, axis=0)
+    assert math.isclose(np.max(np.abs(ut) ** 2), 0.881374 * 0.1 / 185e-15 / 80.5e6, rel_tol=1e-12)
+
+
+def test_band_mask_and_taylor_basis():
+    sim = fiberhip.get_disp_sim_params(1550e-9, 1, 1024, 10.0, 3)
+    m = fiberhip.frequency_band_mask(sim, (-16.0, -10.0))
+    f = fiberhip.fftfreq(1024, 1 / sim["Δt"])
+    assert m.sum() > 0 and np.all(f[m] >= -16.0) and np.all(f[m] <= -10.0)
+    with pytest.raises(ValueError):
+        fiberhip.frequency_band_mask(sim, (5.0, 5.0))
+    b = fiberhip.taylor_phase_basis(sim, (2, 3, 4))
+    osim = R.sim_params(1550e-9, 1, 1024, 10.0, 3)
+    assert np.allclose(b, R.taylor_phase_basis(osim, (2, 3, 4)), rtol=1e-15, atol=0)
+    for bad in ((), (-1, 2), (2, 2)):
+        with pytest.raises(ValueError):
+            fiberhip.taylor_phase_basis(sim, bad)
+
+
+def test_problem_validation_errors():
+    """fiber_field_problem mirrors the reference's ArgumentErrors."""
+    sim = fiberhip.get_disp_sim_params(1550e-9, 1, 64, 3.0, 2)
+    fib = fiberhip.get_disp_fiber_params_user_defined(0.01, sim, gamma_user=1.0, betas_user=[0.0])
+    u = np.ones((64, 1), complex)
+    fiberhip.fiber_field_problem(u, fib, sim)
+    with pytest.raises(ValueError):
+        fiberhip.fiber_field_problem(np.ones((32, 1)), fib, sim)
+    bad = dict(fib)
+    bad["γ"] = np.full((1, 1, 1, 1), 1.0 + 0j)
+    with pytest.raises(ValueError, match="real-valued"):
+        fiberhip.fiber_field_problem(u, bad, sim)
+    with pytest.raises(ValueError):
+        fiberhip.raman_fields(sim, fR=1.1)
+    with pytest.raises(ValueError):
+        fiberhip.raman_fields(sim, tau1=0.0)
+    with pytest.raises(ValueError):
+        fiberhip.get_disp_fiber_params_user_defined(0.01, sim, gamma_user=1.0)
+    with pytest.raises(ValueError):
+        fiberhip.get_disp_sim_params(1550e-9, 1, 63, 3.0, 2)
+    with pytest.raises(ValueError):
+        fiberhip.get_disp_sim_params(1550e-9, 1, 64, 3.0, 1)
+
+
+def _declared_functions() -> list[str]:
+    text = _lib.HEADER_PATH.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(fh_\w+)\s*\(", text, re.M)))
+
+
+def test_library_loads_and_exports_header_symbols():
+    lib = fiberhip.lib()
+    declared = _declared_functions()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(_lib.exported_names()) == declared
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(fh_\w+)", out))
+    assert set(declared) <= exported
+
+
+def test_struct_layout_matches_header():
+    """ProblemDesc: 6 int32 then length_m (8-aligned), 4 pointers, a double."""
+    assert ctypes.sizeof(_lib.ProblemDesc) == 6 * 4 + 8 + 4 * 8 + 8
+    assert _lib.ProblemDesc.length_m.offset == 24
+    assert ctypes.sizeof(_lib.Stats) == 32
+
+
+def test_last_error_starts_empty_and_bad_args_fail_without_gpu():
+    lib = fiberhip.lib()
+    assert lib.fh_last_error() is not None
+    # NULL out-pointer is rejected before any device work
+    assert lib.fh_context_create(0, None) == _lib.FH_EINVAL
+    assert b"NULL" in lib.fh_last_error()
+    assert lib.fh_plan_create(None, None, None) == _lib.FH_EINVAL
