@@ -61,31 +61,21 @@ def launches(problem, batch: int, seed: int = 3) -> np.ndarray:
 
 
 def time_path(problem, steps, precision, batch, k, warmup):
-    import torch
-
+    """K event-timed launches (hipEvents on the launch stream, inside libfiberhip)."""
     import fiberhip
 
     plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch)
-    dt = torch.complex128 if precision == "fp64" else torch.complex64
-    u0 = torch.from_numpy(launches(problem, batch)).to(dt).cuda()
-    out = torch.empty_like(u0)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
+    u0 = plan.to_device(launches(problem, batch)[:, :, None])
+    out = fiberhip.DeviceArray(plan.context, u0.nbytes)
     for _ in range(warmup):
-        plan.forward_device(batch, u0.data_ptr(), out.data_ptr(), sp)
-    torch.cuda.synchronize()
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+        plan.forward_device(batch, u0.ptr, out.ptr)
+    plan.synchronize()
+    print(f"bench: {precision} warmup done, timing {k} launches of {batch} solves", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    for a, b in events:
-        a.record(stream)
-        plan.forward_device(batch, u0.data_ptr(), out.data_ptr(), sp)
-        b.record(stream)
-        print(f"bench: {precision} launch of {batch} solves enqueued", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
+    ms = plan.time_forward_device(batch, u0.ptr, out.ptr, k)
     wall = time.perf_counter() - t0
-    ms = [a.elapsed_time(b) for a, b in events]
-    energies = plan.energy_device(batch, out.data_ptr())
-    return dict(plan=plan, out=out, wall_s=wall, kernel_ms=float(np.mean(ms)), kernel_ms_p50=float(np.median(ms)),
+    energies = plan.energy_device(batch, out.ptr)
+    return dict(plan=plan, wall_s=wall, kernel_ms=float(np.mean(ms)), kernel_ms_p50=float(np.median(ms)),
                 energies=energies)
 
 
@@ -134,14 +124,25 @@ def main() -> None:
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--workload", default="config0")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import ctypes
+
+        import fiberhip
+
+        ctx = fiberhip.default_context()
+        nbytes = args.copy_calibration_mib << 20
+        a, b = fiberhip.DeviceArray(ctx, nbytes), fiberhip.DeviceArray(ctx, nbytes)
+        fiberhip._lib.check(fiberhip.lib().fh_device_copy(ctx.handle, ctypes.c_void_p(b.ptr),
+                                                          ctypes.c_void_p(a.ptr), nbytes))
+        print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (independent solves shard across GPUs without "
               "communication; see DESIGN.md); run with --gpus 1", file=sys.stderr)
         raise SystemExit(2)
-    import torch
-
-    torch.cuda.init()
     problem, steps = workload(args.workload)
     nt, m = problem.nt, problem.modes
     res = {}

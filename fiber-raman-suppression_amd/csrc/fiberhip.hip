@@ -174,6 +174,11 @@ __global__ __launch_bounds__(256) void k_energy(const fh::C2<R>* __restrict__ u,
     if (threadIdx.x == 0) e[blockIdx.x] = red[0];
 }
 
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 int check_desc(const fh_problem_desc* d)
 {
     if (!d) return fail(FH_EINVAL, "problem descriptor is NULL");
@@ -398,6 +403,89 @@ int fh_forward(fh_plan p, int32_t batch, const double* u0, double* uL, fh_stats*
     }
     if (bad) return fail(FH_ENONFINITE, "fixed-step propagation produced non-finite fields");
     return FH_OK;
+}
+
+int fh_device_alloc(fh_context ctx, int64_t bytes, void** ptr)
+{
+    if (!ctx || !ptr || bytes <= 0) return fail(FH_EINVAL, "context, ptr or size invalid");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipMalloc(ptr, (size_t)bytes));
+    return FH_OK;
+}
+
+int fh_device_free(fh_context ctx, void* ptr)
+{
+    if (!ctx) return fail(FH_EINVAL, "context is NULL");
+    if (!ptr) return FH_OK;
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipFree(ptr));
+    return FH_OK;
+}
+
+int fh_copy_to_device(fh_context ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return fail(FH_EINVAL, "invalid copy arguments");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_copy_to_host(fh_context ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return fail(FH_EINVAL, "invalid copy arguments");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_device_copy(fh_context ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0 || (bytes % 16) != 0 || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(FH_EINVAL, "copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    FH_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>(4096, std::max<size_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)src, (uint4*)dst, n);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_synchronize(fh_context ctx)
+{
+    if (!ctx) return fail(FH_EINVAL, "context is NULL");
+    FH_HIP(hipSetDevice(ctx->device));
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    return FH_OK;
+}
+
+int fh_time_forward_device(fh_plan p, int32_t batch, const void* d_u0, void* d_uL, int32_t launches, float* ms)
+{
+    if (!p || !d_u0 || !d_uL || !ms || launches < 1) return fail(FH_EINVAL, "invalid timing arguments");
+    if (batch < 1 || batch > p->d.max_batch) return fail(FH_EINVAL, "batch must lie in [1, max_batch]");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    std::vector<hipEvent_t> ev(2 * (size_t)launches, nullptr);
+    int rc = FH_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(FH_EDEVICE, "hipEventCreate failed");
+    for (int i = 0; rc == FH_OK && i < launches; ++i) {
+        if (hipEventRecord(ev[2 * i], s) != hipSuccess) rc = fail(FH_EDEVICE, "hipEventRecord failed");
+        if (rc == FH_OK)
+            rc = p->d.precision == FH_PREC_FP64 ? launch_forward<double>(p, batch, d_u0, d_uL, s)
+                                                : launch_forward<float>(p, batch, d_u0, d_uL, s);
+        if (rc == FH_OK && hipEventRecord(ev[2 * i + 1], s) != hipSuccess)
+            rc = fail(FH_EDEVICE, "hipEventRecord failed");
+    }
+    if (rc == FH_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(FH_EDEVICE, "hipStreamSynchronize failed");
+    for (int i = 0; rc == FH_OK && i < launches; ++i)
+        if (hipEventElapsedTime(&ms[i], ev[2 * i], ev[2 * i + 1]) != hipSuccess)
+            rc = fail(FH_EDEVICE, "hipEventElapsedTime failed");
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int fh_energy_device(fh_plan p, int32_t batch, const void* d_u, double* energies)

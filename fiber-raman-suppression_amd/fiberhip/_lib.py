@@ -86,6 +86,14 @@ _SIGNATURES = {
     "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
     "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
+    "fh_device_alloc": ([_VP, ctypes.c_int64, ctypes.POINTER(_VP)], ctypes.c_int),
+    "fh_device_free": ([_VP, _VP], ctypes.c_int),
+    "fh_copy_to_device": ([_VP, _VP, _VP, ctypes.c_int64], ctypes.c_int),
+    "fh_copy_to_host": ([_VP, _VP, _VP, ctypes.c_int64], ctypes.c_int),
+    "fh_synchronize": ([_VP], ctypes.c_int),
+    "fh_device_copy": ([_VP, _VP, _VP, ctypes.c_int64], ctypes.c_int),
+    "fh_time_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)],
+                               ctypes.c_int),
     "fh_last_error": ([], ctypes.c_char_p),
 }
 

@@ -47,6 +47,44 @@ def default_context() -> Context:
     return _DEFAULT
 
 
+class DeviceArray:
+    """A device buffer owned through the C ABI (fh_device_alloc)."""
+
+    def __init__(self, context: Context, nbytes: int):
+        self.context = context
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().fh_device_alloc(context.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, context: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(context, a.nbytes)
+        check(lib().fh_copy_to_device(context.handle, ctypes.c_void_p(d.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                                      a.nbytes))
+        return d
+
+    def to_host(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested view is larger than the buffer")
+        check(lib().fh_copy_to_host(self.context.handle, out.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fh_device_free(self.context.handle, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _column_major(a, dtype) -> np.ndarray:
     """(nt, m, ...) array -> contiguous column-major copy (t fastest)."""
     return np.ascontiguousarray(np.asarray(a, dtype=dtype).T)
@@ -123,6 +161,31 @@ class Plan:
         """Enqueue a forward solve on device-resident fields (pointers as ints)."""
         check(lib().fh_forward_device(self.handle, int(batch), ctypes.c_void_p(d_u0), ctypes.c_void_p(d_uL),
                                       ctypes.c_void_p(stream) if stream else None))
+
+    @property
+    def field_dtype(self):
+        return np.complex128 if self.precision == "fp64" else np.complex64
+
+    def to_device(self, u) -> DeviceArray:
+        """(batch, nt, m) complex -> device buffer in the plan's precision and layout."""
+        u = np.asarray(u)
+        if u.ndim == 2:
+            u = u[None]
+        buf = np.ascontiguousarray(np.transpose(u, (0, 2, 1))).astype(self.field_dtype)
+        return DeviceArray.from_host(self.context, buf)
+
+    def from_device(self, d: DeviceArray, batch: int) -> np.ndarray:
+        a = d.to_host(self.field_dtype, (batch, self.modes, self.nt))
+        return np.transpose(a, (0, 2, 1)).astype(np.complex128)
+
+    def time_forward_device(self, batch: int, d_u0: int, d_uL: int, launches: int) -> np.ndarray:
+        ms = (ctypes.c_float * launches)()
+        check(lib().fh_time_forward_device(self.handle, int(batch), ctypes.c_void_p(d_u0), ctypes.c_void_p(d_uL),
+                                           int(launches), ms))
+        return np.array(ms[:], dtype=np.float64)
+
+    def synchronize(self) -> None:
+        check(lib().fh_synchronize(self.context.handle))
 
     def energy_device(self, batch: int, d_u: int) -> np.ndarray:
         out = np.zeros(batch * self.modes)

@@ -119,6 +119,24 @@ int fh_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL,
  * Parseval factor nt*dt). */
 int fh_energy_device(fh_plan plan, int32_t batch, const void *d_u, double *energies);
 
+/* Device buffers on the context's device (plain hipMalloc / hipMemcpy on the
+ * context stream).  Callers without their own HIP allocator use these to
+ * keep fields resident between calls. */
+int fh_device_alloc(fh_context ctx, int64_t bytes, void **ptr);
+int fh_device_free(fh_context ctx, void *ptr);
+int fh_copy_to_device(fh_context ctx, void *dst, const void *src, int64_t bytes);
+int fh_copy_to_host(fh_context ctx, void *dst, const void *src, int64_t bytes);
+int fh_synchronize(fh_context ctx);
+/* Device-to-device copy by a plain streaming kernel (16-byte vector loads and
+ * stores, no reuse); also the reference point for HBM byte counters. */
+int fh_device_copy(fh_context ctx, void *dst, const void *src, int64_t bytes);
+
+/* Enqueue `launches` back-to-back forward solves of `batch` device-resident
+ * launches on the context stream, each bracketed by a hipEvent pair on that
+ * stream, wait for them, and return each launch's duration in ms_out[i]. */
+int fh_time_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, int32_t launches,
+                           float *ms_out);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *fh_last_error(void);
 
