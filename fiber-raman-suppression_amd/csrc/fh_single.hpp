@@ -1,20 +1,30 @@
-// fh_single.hpp -- fused single-mode RK4IP propagation (one workgroup per solve).
+// fh_single.hpp -- fused single-mode RK4IP propagation and its discrete
+// adjoint, one workgroup per solve.
 //
 // The whole fixed-step integration of one solve runs inside one workgroup:
-// the field stays in registers (cyclic layout, see fh_device.hpp) and in a
+// the field stays in registers (cyclic layout, see fh_device.hpp) or in a
 // workgroup-private slice of global scratch, every FFT goes through LDS, and
-// HBM only sees the launch spectrum, the output spectrum and the shared
-// operator tables (D, hR, shock, twiddles), which stay L2-resident across the
-// batch.  A launch of B solves uses B workgroups.
+// HBM only sees the launch, the outputs, checkpoints/tape of the adjoint and
+// the shared operator tables (D, hR, shock, twiddles), which stay L2-resident
+// across the batch.  A launch of B solves uses B workgroups.
 //
 // Right-hand side, restating disp_mmf! for M = 1 in the operation order of
 // _scalar_rk4_output's closure (scalar_crosscheck.jl:38-48):
 //   u   = fft(cis(D z) .* x)
 //   q   = gamma |u|^2
 //   dR  = real(fftshift(ifft(hR .* fft(q))))   -- fftshift folded into hR as (-1)^k
-//   eta = ifft(((1-fR) q + dR) .* u) .* shock
+//   h   = (1-fR) q + dR
+//   eta = ifft(h .* u) .* shock
 //   dx  = i cis(-D z) .* eta
 // and the classic global-frame RK4 of scalar_crosscheck.jl:53-60.
+//
+// Pullback of the right-hand side (real-linear map; cotangent gy of dx):
+//   G  = fft(-i cis(D z) (shock/N) gy)
+//   gu = h G + 2 gamma b u,   a = Re(conj(G) u),  b = (1-fR) a + real(ifft(conj(hR)(-1)^k fft(a)))
+//   gx = conj(cis(D z)) N ifft(gu)
+// with u and h of the stage taken from the tape recorded by the segment
+// recomputation.  conj(hR) and the (-1)^k alternation are the same operators
+// adjoint_disp_mmf! uses (sensitivity_disp_mmf.jl:96,127,241-242).
 #pragma once
 #include "fh_device.hpp"
 
@@ -32,13 +42,23 @@ template <typename R> struct SmArgs {
     R gamma, one_m_fr;
     double h, length;
     int steps;
+    // adjoint only
+    const C2<R>* seed;          // [batch][N] lab-frame Wirtinger seed, or null for the band objective
+    const unsigned char* mask;  // [N] band mask of the band objective
+    int log_cost;               // 10 log10 J (physics_helpers.jl:320-326)
+    C2<R>* lam0;                // [batch][N] dJ/dconj(u0)
+    double* grad;               // [batch][N] 2 Re(conj(lam0) i u0)
+    double* cost;               // [batch]
+    C2<R>* ckpt;                // [batch][nseg][N]
+    C2<R>* tape_u;              // [batch][4*every][N]
+    R* tape_h;                  // [batch][4*every][N]
+    int every, nseg;
 };
 
 template <typename R, int N, int T> struct SmTraits;
-// FP64: state/accumulator in global scratch; time field in registers when a
-// workgroup has at most 512 threads (2 waves per SIMD -> 256 VGPRs each);
-// the rotation cis(D z) is re-evaluated for the back-rotation instead of
-// being held across the four transforms.
+// FP64: state/accumulators in global scratch; time field in registers when a
+// workgroup has at most 512 threads and 8 samples per thread; the rotation
+// cis(D z) is re-evaluated for the back-rotation instead of being held.
 template <int N, int T> struct SmTraits<double, N, T> {
     static constexpr bool STATE_REGS = false;
     static constexpr bool UT_REGS = T <= 512 && N / T <= 8;
@@ -54,8 +74,25 @@ template <int N, int T> struct SmTraits<float, N, T> {
 
 template <typename R, int N, int T> struct SmLayout {
     using TR = SmTraits<R, N, T>;
-    static constexpr int NW = (TR::STATE_REGS ? 0 : 2) + (TR::UT_REGS ? 0 : 1);
+    // st, ac, lam, lacc when not in registers; ut, gu when not in registers
+    static constexpr int NW = (TR::STATE_REGS ? 0 : 4) + (TR::UT_REGS ? 0 : 2);
+    static constexpr int NWA = NW > 0 ? NW : 1;
     static constexpr size_t LDS_BYTES = (size_t)LdsSize<N>::value * (TR::PLANAR ? sizeof(R) : 2 * sizeof(R));
+};
+
+// A cyclic field of this thread, in registers or in a global slice.
+template <typename V, int T, int P, bool REGS> struct Field;
+template <typename V, int T, int P> struct Field<V, T, P, true> {
+    V r[P];
+    __device__ __forceinline__ explicit Field(const Buf&) {}
+    __device__ __forceinline__ V get(int k) const { return r[k]; }
+    __device__ __forceinline__ void set(int k, V x) { r[k] = x; }
+};
+template <typename V, int T, int P> struct Field<V, T, P, false> {
+    Buf b;
+    __device__ __forceinline__ explicit Field(const Buf& bb) : b(bb) {}
+    __device__ __forceinline__ V get(int k) const { return b.ld<V>(Cyc<V, T>::voff(), Cyc<V, T>::soff(k)); }
+    __device__ __forceinline__ void set(int k, V x) { b.st(x, Cyc<V, T>::voff(), Cyc<V, T>::soff(k)); }
 };
 
 // Buffer views of the shared operator tables.
@@ -68,11 +105,21 @@ template <typename R, int N> struct SmTables {
     }
 };
 
+template <typename R, int N, int T> struct SmCtx {
+    using V = C2<R>;
+    using TR = SmTraits<R, N, T>;
+    using LY = SmLayout<R, N, T>;
+    static constexpr int P = N / T;
+    using FS = Field<V, T, P, TR::STATE_REGS>;
+    using FU = Field<V, T, P, TR::UT_REGS>;
+};
+
 // One right-hand-side evaluation.  On entry v holds the stage input y
-// (interaction picture, cyclic layout); on exit v holds f(zs, y).
-template <typename R, int N, int T>
+// (interaction picture, cyclic layout); on exit v holds f(zs, y).  With TAPE
+// the stage's time field u and weight h are stored to tu / th.
+template <typename R, int N, int T, bool TAPE>
 __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, const SmTables<R, N>& tb,
-                                       const Buf& gut, void* lds)
+                                       typename SmCtx<R, N, T>::FU& ut, const Buf& tu, const Buf& th, void* lds)
 {
     using V = C2<R>;
     using TR = SmTraits<R, N, T>;
@@ -81,7 +128,6 @@ __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, 
     using CR = Cyc<R, T>;
     constexpr int P = N / T;
     R cs[TR::KEEP_PHASE ? P : 1], sn[TR::KEEP_PHASE ? P : 1];
-    V ut[TR::UT_REGS ? P : 1];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         R c, s;
@@ -95,10 +141,8 @@ __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, 
     block_fft<R, N, T, -1, TR::PLANAR>(v, tb.tw, lds);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        if constexpr (TR::UT_REGS)
-            ut[k] = v[k];
-        else
-            gut.st(v[k], CV::voff(), CV::soff(k));
+        ut.set(k, v[k]);
+        if constexpr (TAPE) tu.st(v[k], CV::voff(), CV::soff(k));
         const R q = a.gamma * (v[k].x * v[k].x + v[k].y * v[k].y);
         v[k] = V{q, (R)0};
     }
@@ -108,13 +152,10 @@ __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, 
     block_fft<R, N, T, +1, TR::PLANAR>(v, tb.tw, lds);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        V u;
-        if constexpr (TR::UT_REGS)
-            u = ut[k];
-        else
-            u = gut.ld<V>(CV::voff(), CV::soff(k));
+        const V u = ut.get(k);
         const R q = a.gamma * (u.x * u.x + u.y * u.y);
         const R f = a.one_m_fr * q + v[k].x;
+        if constexpr (TAPE) th.st(f, CR::voff(), CR::soff(k));
         v[k] = V{u.x * f, u.y * f};
     }
     block_fft<R, N, T, +1, TR::PLANAR>(v, tb.tw, lds);
@@ -134,79 +175,71 @@ __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, 
     }
 }
 
+// One RK4 step from state st (v holds st on entry and the new state on exit).
+template <typename R, int N, int T, bool TAPE>
+__device__ __forceinline__ void sm_step(C2<R>* v, int step, const SmArgs<R>& a, const SmTables<R, N>& tb,
+                                        typename SmCtx<R, N, T>::FS& st, typename SmCtx<R, N, T>::FS& ac,
+                                        typename SmCtx<R, N, T>::FU& ut, C2<R>* tape_u, R* tape_h, void* lds)
+{
+    using V = C2<R>;
+    constexpr int P = N / T;
+    const double h = a.h;
+    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
+    const double z = step * h;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
+        const Buf tu(TAPE ? tape_u + (size_t)s * N : tape_u, N * sizeof(V));
+        const Buf th(TAPE ? tape_h + (size_t)s * N : tape_h, N * sizeof(R));
+        sm_rhs<R, N, T, TAPE>(v, zs, a, tb, ut, tu, th, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V x = st.get(k);
+            V c = s != 0 ? ac.get(k) : V{(R)0, (R)0};
+            const V kk = v[k];
+            if (s == 0) {
+                c = kk;
+                v[k] = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+            } else if (s == 1) {
+                c = V{c.x + 2 * kk.x, c.y + 2 * kk.y};
+                v[k] = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+            } else if (s == 2) {
+                c = V{c.x + 2 * kk.x, c.y + 2 * kk.y};
+                v[k] = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
+            } else {
+                const V t = V{c.x + kk.x, c.y + kk.y};
+                const V xn = V{x.x + h6 * t.x, x.y + h6 * t.y};
+                v[k] = xn;
+                st.set(k, xn);
+            }
+            if (s != 3) ac.set(k, c);
+        }
+    }
+}
+
 template <typename R, int N, int T>
 __global__ __launch_bounds__(T) void k_rk4ip_single(SmArgs<R> a)
 {
+    using CX = SmCtx<R, N, T>;
     using V = C2<R>;
-    using TR = SmTraits<R, N, T>;
-    using LY = SmLayout<R, N, T>;
+    using LY = typename CX::LY;
     using CV = Cyc<V, T>;
     constexpr int P = N / T;
     __shared__ __align__(16) unsigned char lds[LY::LDS_BYTES];
     const size_t sol = blockIdx.x;
     const SmTables<R, N> tb(a, FftPlan<N, T>::TW_SIZE);
     const Buf u0(a.u0 + sol * N, N * sizeof(V));
-    V* gwork = a.work + sol * (size_t)(LY::NW > 0 ? LY::NW : 1) * N;
-    const Buf gst(gwork, N * sizeof(V));
-    const Buf gac(gwork + N, N * sizeof(V));
-    const Buf gut(gwork + (TR::STATE_REGS ? 0 : 2 * N), N * sizeof(V));
+    V* gwork = a.work + sol * (size_t)LY::NWA * N;
+    typename CX::FS st(Buf(gwork, N * sizeof(V))), ac(Buf(gwork + N, N * sizeof(V)));
+    typename CX::FU ut(Buf(gwork + (size_t)(CX::TR::STATE_REGS ? 0 : 4) * N, N * sizeof(V)));
 
     V v[P];
-    V st[TR::STATE_REGS ? P : 1], ac[TR::STATE_REGS ? P : 1];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const V x = u0.ld<V>(CV::voff(), CV::soff(k));
-        v[k] = x;
-        if constexpr (TR::STATE_REGS)
-            st[k] = x;
-        else
-            gst.st(x, CV::voff(), CV::soff(k));
+        v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
+        st.set(k, v[k]);
     }
-    const double h = a.h;
-    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
-    for (int step = 0; step < a.steps; ++step) {
-        const double z = step * h;
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s) {
-            const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
-            sm_rhs<R, N, T>(v, zs, a, tb, gut, lds);
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                V x, c;
-                if constexpr (TR::STATE_REGS) {
-                    x = st[k];
-                    c = ac[k];
-                } else {
-                    x = gst.ld<V>(CV::voff(), CV::soff(k));
-                    c = s != 0 ? gac.ld<V>(CV::voff(), CV::soff(k)) : V{(R)0, (R)0};
-                }
-                const V kk = v[k];
-                if (s == 0) {
-                    c = kk;
-                    v[k] = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-                } else if (s == 1) {
-                    c = V{c.x + 2 * kk.x, c.y + 2 * kk.y};
-                    v[k] = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-                } else if (s == 2) {
-                    c = V{c.x + 2 * kk.x, c.y + 2 * kk.y};
-                    v[k] = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
-                } else {
-                    const V t = V{c.x + kk.x, c.y + kk.y};
-                    x = V{x.x + h6 * t.x, x.y + h6 * t.y};
-                    v[k] = x;
-                }
-                if constexpr (TR::STATE_REGS) {
-                    if (s == 3) st[k] = x;
-                    ac[k] = c;
-                } else {
-                    if (s == 3)
-                        gst.st(x, CV::voff(), CV::soff(k));
-                    else
-                        gac.st(c, CV::voff(), CV::soff(k));
-                }
-            }
-        }
-    }
+    for (int step = 0; step < a.steps; ++step) sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, nullptr, nullptr, lds);
     int bad = 0;
     const Buf out(a.out + sol * N, N * sizeof(V));
     using CD = Cyc<double, T>;
@@ -218,6 +251,231 @@ __global__ __launch_bounds__(T) void k_rk4ip_single(SmArgs<R> a)
         const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
         bad |= !(isfinite(o.x) && isfinite(o.y));
         out.st(o, CV::voff(), CV::soff(k));
+    }
+    if (bad) atomicOr(a.status + sol, 1);
+}
+
+// Pullback of one right-hand-side evaluation.  On entry v holds the cotangent
+// gy of the stage output, on exit the cotangent gx of the stage input.
+template <typename R, int N, int T>
+__device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, const SmTables<R, N>& tb,
+                                       typename SmCtx<R, N, T>::FU& gu, const Buf& tu, const Buf& th, void* lds)
+{
+    using V = C2<R>;
+    using TR = SmTraits<R, N, T>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    using CR = Cyc<R, T>;
+    constexpr int P = N / T;
+    R cs[TR::KEEP_PHASE ? P : 1], sn[TR::KEEP_PHASE ? P : 1];
+    R av[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        R c, s;
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), zs, c, s);
+        if constexpr (TR::KEEP_PHASE) {
+            cs[k] = c;
+            sn[k] = s;
+        }
+        const R sc = tb.shn.template ld<R>(CR::voff(), CR::soff(k));
+        const V e = V{v[k].x * sc, v[k].y * sc};
+        const V w = V{e.x * c - e.y * s, e.x * s + e.y * c};
+        v[k] = V{w.y, -w.x};  // -i w
+    }
+    block_fft<R, N, T, -1, TR::PLANAR>(v, tb.tw, lds);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const V u = tu.ld<V>(CV::voff(), CV::soff(k));
+        const R hh = th.ld<R>(CR::voff(), CR::soff(k));
+        const V G = v[k];
+        gu.set(k, V{hh * G.x, hh * G.y});
+        av[k] = G.x * u.x + G.y * u.y;
+        v[k] = V{av[k], (R)0};
+    }
+    block_fft<R, N, T, -1, TR::PLANAR>(v, tb.tw, lds);
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = cmul(v[k], cconj(tb.hn.template ld<V>(CV::voff(), CV::soff(k))));
+    block_fft<R, N, T, +1, TR::PLANAR>(v, tb.tw, lds);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const V u = tu.ld<V>(CV::voff(), CV::soff(k));
+        const R b = v[k].x + a.one_m_fr * av[k];
+        const R f = 2 * a.gamma * b;
+        const V g = gu.get(k);
+        v[k] = V{g.x + f * u.x, g.y + f * u.y};
+    }
+    block_fft<R, N, T, +1, TR::PLANAR>(v, tb.tw, lds);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        R c, s;
+        if constexpr (TR::KEEP_PHASE) {
+            c = cs[k];
+            s = sn[k];
+        } else {
+            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), zs, c, s);
+        }
+        v[k] = V{c * v[k].x + s * v[k].y, c * v[k].y - s * v[k].x};  // conj(cis) * v
+    }
+}
+
+// Workgroup FP64 sum (deterministic tree) through the LDS buffer.
+template <int T> __device__ __forceinline__ double block_sum(double x, void* lds_raw)
+{
+    double* red = reinterpret_cast<double*>(lds_raw);
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = x;
+    __syncthreads();
+#pragma unroll 1
+    for (int w = T / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Forward solve with checkpoints, terminal seed (caller-supplied or the band
+// objective of _field_spectral_band_cost, physics_helpers.jl:222-232),
+// backward sweep of the discrete adjoint with per-segment recomputation, and
+// the phase chain rule g = 2 Re(conj(lam0) i u0) (physics_models.jl:1042) in
+// FP64.
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
+{
+    using CX = SmCtx<R, N, T>;
+    using V = C2<R>;
+    using LY = typename CX::LY;
+    using TR = typename CX::TR;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = N / T;
+    __shared__ __align__(16) unsigned char lds[LY::LDS_BYTES];
+    const size_t sol = blockIdx.x;
+    const SmTables<R, N> tb(a, FftPlan<N, T>::TW_SIZE);
+    const Buf u0(a.u0 + sol * N, N * sizeof(V));
+    V* gwork = a.work + sol * (size_t)LY::NWA * N;
+    constexpr int US = TR::STATE_REGS ? 0 : 4;
+    typename CX::FS st(Buf(gwork, N * sizeof(V))), ac(Buf(gwork + N, N * sizeof(V)));
+    typename CX::FS lam(Buf(gwork + 2 * N, N * sizeof(V))), lacc(Buf(gwork + 3 * N, N * sizeof(V)));
+    typename CX::FU ut(Buf(gwork + (size_t)US * N, N * sizeof(V)));
+    typename CX::FU gu(Buf(gwork + (size_t)(US + 1) * N, N * sizeof(V)));
+    V* ckpt = a.ckpt + sol * (size_t)a.nseg * N;
+    V* tape_u = a.tape_u + sol * (size_t)4 * a.every * N;
+    R* tape_h = a.tape_h + sol * (size_t)4 * a.every * N;
+    const double h = a.h;
+
+    // forward with checkpoints
+    V v[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
+        st.set(k, v[k]);
+    }
+    for (int step = 0; step < a.steps; ++step) {
+        if (step % a.every == 0) {
+            const Buf ck(ckpt + (size_t)(step / a.every) * N, N * sizeof(V));
+#pragma unroll
+            for (int k = 0; k < P; ++k) ck.st(v[k], CV::voff(), CV::soff(k));
+        }
+        sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, nullptr, nullptr, lds);
+    }
+    // output and terminal seed
+    const Buf out(a.out + sol * N, N * sizeof(V));
+    double etot = 0.0, eband = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        R c, s;
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length, c, s);
+        const V x = v[k];
+        const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
+        bad |= !(isfinite(o.x) && isfinite(o.y));
+        out.st(o, CV::voff(), CV::soff(k));
+        v[k] = o;
+        const double e = (double)o.x * o.x + (double)o.y * o.y;
+        etot += e;
+        if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+    }
+    if (!a.seed) {
+        etot = block_sum<T>(etot, lds);
+        eband = block_sum<T>(eband, lds);
+        const double J = eband / etot;
+        double scale = 1.0 / etot;
+        if (a.log_cost) scale *= J <= 1e-15 ? 0.0 : 10.0 / (2.302585092994045684 * J);
+        if (threadIdx.x == 0) a.cost[sol] = a.log_cost ? 10.0 * log10(fmax(J, 1e-15)) : J;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const R f = (R)(((a.mask[threadIdx.x + T * k] ? 1.0 : 0.0) - J) * scale);
+            v[k] = V{v[k].x * f, v[k].y * f};
+        }
+    } else {
+        const Buf sd(a.seed + sol * N, N * sizeof(V));
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = sd.ld<V>(CV::voff(), CV::soff(k));
+    }
+    // interaction picture: lam = cis(-D L) lamL
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        R c, s;
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length, c, s);
+        lam.set(k, V{c * v[k].x + s * v[k].y, c * v[k].y - s * v[k].x});
+    }
+    const R c6 = (R)(h / 6), c3 = (R)(h / 3), c2 = (R)(h / 2), c1 = (R)h;
+    for (int sg = a.nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * a.every;
+        const int s1 = min(s0 + a.every, a.steps);
+        const Buf ck(ckpt + (size_t)sg * N, N * sizeof(V));
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            v[k] = ck.ld<V>(CV::voff(), CV::soff(k));
+            st.set(k, v[k]);
+        }
+        for (int step = s0; step < s1; ++step)
+            sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u + (size_t)4 * (step - s0) * N,
+                                   tape_h + (size_t)4 * (step - s0) * N, lds);
+        for (int step = s1 - 1; step >= s0; --step) {
+            const double z = step * h;
+            V* tus = tape_u + (size_t)4 * (step - s0) * N;
+            R* ths = tape_h + (size_t)4 * (step - s0) * N;
+#pragma unroll 1
+            for (int s = 3; s >= 0; --s) {
+                const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
+                const R cl = (s == 0 || s == 3) ? c6 : c3;   // weight of k_s in the update
+                const R cg = s == 3 ? (R)0 : (s == 2 ? c1 : c2);  // dy_{s+1}/dk_s
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    const V l = lam.get(k);
+                    if (s == 3) lacc.set(k, l);
+                    v[k] = V{cl * l.x + cg * v[k].x, cl * l.y + cg * v[k].y};
+                }
+                const Buf tu(tus + (size_t)s * N, N * sizeof(V));
+                const Buf th(ths + (size_t)s * N, N * sizeof(R));
+                sm_vjp<R, N, T>(v, zs, a, tb, gu, tu, th, lds);
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    const V c = lacc.get(k);
+                    const V nc = V{c.x + v[k].x, c.y + v[k].y};
+                    if (s == 0)
+                        lam.set(k, nc);
+                    else
+                        lacc.set(k, nc);
+                }
+            }
+        }
+    }
+    // lam0 and the phase chain rule in FP64
+    const Buf l0(a.lam0 + sol * N, N * sizeof(V));
+    const Buf gr(a.grad + sol * N, N * sizeof(double));
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const V l = lam.get(k);
+        const V u = u0.ld<V>(CV::voff(), CV::soff(k));
+        l0.st(l, CV::voff(), CV::soff(k));
+        const double g = 2.0 * ((double)l.y * (double)u.x - (double)l.x * (double)u.y);
+        bad |= !(isfinite(l.x) && isfinite(l.y));
+        gr.st(g, CD::voff(), CD::soff(k));
     }
     if (bad) atomicOr(a.status + sol, 1);
 }

@@ -47,6 +47,9 @@ struct fh_plan_s {
     fh_problem_desc d;
     int threads;  // workgroup size of the single-mode kernel
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
+    DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
+    int every = 50, nseg = 0;
+    bool has_mask = false;
     size_t work_per_solve = 0;  // complex entries
     int64_t total_bytes = 0;
     double gamma0 = 0.0;        // gamma[1,1,1,1] of a single-mode plan
@@ -115,6 +118,15 @@ template <typename R, int N> struct SingleKernel {
     }
 };
 
+template <typename R, int N> struct GradKernel {
+    static constexpr int T = SingleShape<R, N>::T;
+    static hipError_t launch(const fh::SmArgs<R>& a, int batch, hipStream_t s)
+    {
+        hipLaunchKernelGGL((fh::k_grad_single<R, N, T>), dim3(batch), dim3(T), 0, s, a);
+        return hipGetLastError();
+    }
+};
+
 template <typename R, template <typename, int> class Op, typename... A>
 auto with_size(int nt, A&&... args) -> decltype(Op<R, 1024>::run(args...))
 {
@@ -143,6 +155,10 @@ template <typename R, int N> struct OpTw {
         return SingleKernel<R, N>::T;
     }
     static int unsupported() { return 0; }
+};
+template <typename R, int N> struct OpGrad {
+    static hipError_t run(const fh::SmArgs<R>& a, int batch, hipStream_t s) { return GradKernel<R, N>::launch(a, batch, s); }
+    static hipError_t unsupported() { return hipErrorInvalidValue; }
 };
 template <typename R, int N> struct OpLaunch {
     static hipError_t run(const fh::SmArgs<R>& a, int batch, hipStream_t s)
@@ -210,6 +226,7 @@ int check_desc(const fh_problem_desc* d)
     for (size_t i = 0; i < nt * m; ++i) dmax = std::max(dmax, std::fabs(d->dispersion[i]));
     if (dmax * d->length_m >= FH_MAX_PHASE)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
+    if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->modes != 1) return fail(FH_EUNSUPPORTED, "this build integrates single-mode problems only");
     if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
     return FH_OK;
@@ -243,6 +260,17 @@ template <typename R> int upload_tables(fh_plan p)
     FH_HIP(hipMemcpy(p->hn.p, hn.data(), p->hn.bytes, hipMemcpyHostToDevice));
     FH_HIP(hipMemcpy(p->shn.p, shn.data(), p->shn.bytes, hipMemcpyHostToDevice));
     FH_HIP(hipMemcpy(p->tw.p, twr.data(), p->tw.bytes, hipMemcpyHostToDevice));
+    if (d.band_mask) {
+        bool any = false;
+        for (int k = 0; k < nt; ++k) any |= d.band_mask[k] != 0;
+        if (!any) return fail(FH_EINVAL, "band mask must contain at least one bin");
+        if ((rc = dev_alloc(p, p->mask, (size_t)nt))) return rc;
+        FH_HIP(hipMemcpy(p->mask.p, d.band_mask, (size_t)nt, hipMemcpyHostToDevice));
+        p->has_mask = true;
+    }
+    p->every = d.checkpoint_every > 0 ? d.checkpoint_every : 50;
+    if (p->every > d.steps) p->every = d.steps;
+    p->nseg = (d.steps + p->every - 1) / p->every;
     p->work_per_solve = with_size<R, OpWork>(nt);
     const size_t mb = (size_t)d.max_batch;
     if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
@@ -253,12 +281,10 @@ template <typename R> int upload_tables(fh_plan p)
     return FH_OK;
 }
 
-template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+template <typename R> fh::SmArgs<R> base_args(fh_plan p)
 {
     const fh_problem_desc& d = p->d;
-    fh::SmArgs<R> a;
-    a.u0 = (const fh::C2<R>*)d_u0;
-    a.out = (fh::C2<R>*)d_uL;
+    fh::SmArgs<R> a{};
     a.work = (fh::C2<R>*)p->work.p;
     a.disp = (const double*)p->disp.p;
     a.hn = (const fh::C2<R>*)p->hn.p;
@@ -270,6 +296,59 @@ template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0,
     a.h = d.length_m / d.steps;
     a.length = d.length_m;
     a.steps = d.steps;
+    a.every = p->every;
+    a.nseg = p->nseg;
+    a.log_cost = d.log_cost;
+    a.mask = (const unsigned char*)p->mask.p;
+    return a;
+}
+
+template <typename R> int ensure_adjoint_buffers(fh_plan p)
+{
+    if (p->ckpt.p) return FH_OK;
+    const size_t nt = (size_t)p->d.nt, mb = (size_t)p->d.max_batch, V = 2 * sizeof(R);
+    int rc;
+    if ((rc = dev_alloc(p, p->ckpt, V * nt * (size_t)p->nseg * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_u, V * nt * 4 * (size_t)p->every * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, sizeof(R) * nt * 4 * (size_t)p->every * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_lam0, V * nt * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_seed, V * nt * mb))) return rc;
+    return FH_OK;
+}
+
+template <typename R>
+int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
+                    double* d_grad, double* d_cost, hipStream_t s)
+{
+    if (!d_seed && !p->has_mask)
+        return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
+    int rc = ensure_adjoint_buffers<R>(p);
+    if (rc) return rc;
+    fh::SmArgs<R> a = base_args<R>(p);
+    a.u0 = (const fh::C2<R>*)d_u0;
+    a.seed = (const fh::C2<R>*)d_seed;
+    a.out = (fh::C2<R>*)d_uL;
+    a.lam0 = (fh::C2<R>*)d_lam0;
+    a.grad = d_grad;
+    a.cost = d_cost;
+    a.ckpt = (fh::C2<R>*)p->ckpt.p;
+    a.tape_u = (fh::C2<R>*)p->tape_u.p;
+    a.tape_h = (R*)p->tape_h.p;
+    FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    hipError_t e = with_size<R, OpGrad>(p->d.nt, a, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("gradient launch: ") + hipGetErrorString(e));
+    return FH_OK;
+}
+
+template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+{
+    const fh_problem_desc& d = p->d;
+    fh::SmArgs<R> a = base_args<R>(p);
+    (void)d;
+    a.u0 = (const fh::C2<R>*)d_u0;
+    a.out = (fh::C2<R>*)d_uL;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
     hipError_t e = with_size<R, OpLaunch>(d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -321,6 +400,7 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     p->gamma0 = desc->gamma[0];
     rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p) : upload_tables<float>(p);
     p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
+    p->d.band_mask = nullptr;
     if (rc) {
         fh_plan_destroy(p);
         return rc;
@@ -333,7 +413,9 @@ int fh_plan_destroy(fh_plan p)
 {
     if (!p) return FH_OK;
     (void)hipSetDevice(p->ctx->device);
-    for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out}) dev_free(*b);
+    for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out, &p->mask,
+                      &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed})
+        dev_free(*b);
     delete p;
     return FH_OK;
 }
@@ -486,6 +568,118 @@ int fh_time_forward_device(fh_plan p, int32_t batch, const void* d_u0, void* d_u
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     return rc;
+}
+
+int fh_gradient_device(fh_plan p, int32_t batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
+                       double* d_grad, double* d_cost, void* stream)
+{
+    if (!p || !d_u0 || !d_uL || !d_lam0 || !d_grad || !d_cost) return fail(FH_EINVAL, "NULL argument");
+    if (batch < 1 || batch > p->d.max_batch) return fail(FH_EINVAL, "batch must lie in [1, max_batch]");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
+    return p->d.precision == FH_PREC_FP64
+               ? launch_gradient<double>(p, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s)
+               : launch_gradient<float>(p, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s);
+}
+
+int fh_time_gradient_device(fh_plan p, int32_t batch, const void* d_u0, void* d_uL, void* d_lam0, double* d_grad,
+                            double* d_cost, int32_t launches, float* ms)
+{
+    if (!p || !ms || launches < 1) return fail(FH_EINVAL, "invalid timing arguments");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    std::vector<hipEvent_t> ev(2 * (size_t)launches, nullptr);
+    int rc = FH_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(FH_EDEVICE, "hipEventCreate failed");
+    for (int i = 0; rc == FH_OK && i < launches; ++i) {
+        if (hipEventRecord(ev[2 * i], s) != hipSuccess) rc = fail(FH_EDEVICE, "hipEventRecord failed");
+        if (rc == FH_OK) rc = fh_gradient_device(p, batch, d_u0, nullptr, d_uL, d_lam0, d_grad, d_cost, s);
+        if (rc == FH_OK && hipEventRecord(ev[2 * i + 1], s) != hipSuccess)
+            rc = fail(FH_EDEVICE, "hipEventRecord failed");
+    }
+    if (rc == FH_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(FH_EDEVICE, "hipStreamSynchronize failed");
+    for (int i = 0; rc == FH_OK && i < launches; ++i)
+        if (hipEventElapsedTime(&ms[i], ev[2 * i], ev[2 * i + 1]) != hipSuccess)
+            rc = fail(FH_EDEVICE, "hipEventElapsedTime failed");
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int fh_gradient(fh_plan p, int32_t batch, const double* u0, const double* seed, double* uL, double* lam0,
+                double* grad, double* cost, fh_stats* stats)
+{
+    if (!p || !u0) return fail(FH_EINVAL, "plan or launch pointer is NULL");
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    const size_t n = (size_t)batch * p->d.nt * p->d.modes;
+    double norm2 = 0.0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");
+        norm2 += u0[i] * u0[i];
+    }
+    if (!(norm2 > 0)) return fail(FH_EINVAL, "input spectrum must be nonzero");
+    if (seed)
+        for (size_t i = 0; i < 2 * n; ++i)
+            if (!std::isfinite(seed[i])) return fail(FH_EINVAL, "adjoint terminal condition must be finite");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    int rc = fp64 ? ensure_adjoint_buffers<double>(p) : ensure_adjoint_buffers<float>(p);
+    if (rc) return rc;
+    const size_t es = fp64 ? sizeof(double) : sizeof(float);
+    auto to_dev = [&](void* dst, const double* src) -> int {
+        if (fp64) {
+            FH_HIP(hipMemcpyAsync(dst, src, 2 * es * n, hipMemcpyHostToDevice, s));
+        } else {
+            std::vector<float> tmp(2 * n);
+            for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)src[i];
+            FH_HIP(hipMemcpyAsync(dst, tmp.data(), 2 * es * n, hipMemcpyHostToDevice, s));
+            FH_HIP(hipStreamSynchronize(s));
+        }
+        return FH_OK;
+    };
+    auto to_host = [&](double* dst, const void* src) -> int {
+        if (fp64) {
+            FH_HIP(hipMemcpyAsync(dst, src, 2 * es * n, hipMemcpyDeviceToHost, s));
+            FH_HIP(hipStreamSynchronize(s));
+        } else {
+            std::vector<float> tmp(2 * n);
+            FH_HIP(hipMemcpyAsync(tmp.data(), src, 2 * es * n, hipMemcpyDeviceToHost, s));
+            FH_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < 2 * n; ++i) dst[i] = tmp[i];
+        }
+        return FH_OK;
+    };
+    if ((rc = to_dev(p->u_in.p, u0))) return rc;
+    if (seed && (rc = to_dev(p->g_seed.p, seed))) return rc;
+    rc = fh_gradient_device(p, batch, p->u_in.p, seed ? p->g_seed.p : nullptr, p->u_out.p, p->g_lam0.p,
+                            (double*)p->g_grad.p, (double*)p->g_cost.p, s);
+    if (rc) return rc;
+    std::vector<int> status(batch);
+    FH_HIP(hipMemcpyAsync(status.data(), p->status.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (uL && (rc = to_host(uL, p->u_out.p))) return rc;
+    if (lam0 && (rc = to_host(lam0, p->g_lam0.p))) return rc;
+    const size_t nb = (size_t)batch * p->d.nt;
+    if (grad) FH_HIP(hipMemcpy(grad, p->g_grad.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    if (cost) {
+        if (seed)
+            for (int b = 0; b < batch; ++b) cost[b] = NAN;
+        else
+            FH_HIP(hipMemcpy(cost, p->g_cost.p, sizeof(double) * batch, hipMemcpyDeviceToHost));
+    }
+    int bad = 0;
+    for (int b = 0; b < batch; ++b) bad |= status[b];
+    if (stats) {
+        stats->accepted = 2LL * p->d.steps;
+        stats->rejected = 0;
+        stats->rhs_evals = 8LL * p->d.steps;
+        stats->retcode = bad ? FH_ENONFINITE : FH_OK;
+    }
+    if (bad) return fail(FH_ENONFINITE, "adjoint propagation produced non-finite fields");
+    return FH_OK;
 }
 
 int fh_energy_device(fh_plan p, int32_t batch, const void* d_u, double* energies)

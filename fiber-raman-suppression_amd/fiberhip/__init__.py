@@ -28,6 +28,18 @@ from .problem import (  # noqa: F401
     with_launch,
     with_raman_fraction,
 )
+from .adjoint import (  # noqa: F401
+    AdjointModel,
+    FullGridPhase,
+    ObjectiveMap,
+    PhaseBasis,
+    band_gradient,
+    check_adjoint_gradient,
+    fiber_model,
+    raman_band_objective,
+    run_adjoint_step,
+    spectral_band_objective,
+)
 from .propagation import (  # noqa: F401
     PropagationResult,
     centroid_thz,

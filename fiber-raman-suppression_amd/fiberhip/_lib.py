@@ -60,6 +60,9 @@ class ProblemDesc(ctypes.Structure):
         ("raman", ctypes.POINTER(ctypes.c_double)),
         ("shock", ctypes.POINTER(ctypes.c_double)),
         ("one_m_fr", ctypes.c_double),
+        ("checkpoint_every", ctypes.c_int32),
+        ("log_cost", ctypes.c_int32),
+        ("band_mask", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 
@@ -86,6 +89,10 @@ _SIGNATURES = {
     "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
     "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
+    "fh_gradient": ([_VP, ctypes.c_int32, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "fh_gradient_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "fh_time_gradient_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, ctypes.c_int32,
+                                 ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "fh_device_alloc": ([_VP, ctypes.c_int64, ctypes.POINTER(_VP)], ctypes.c_int),
     "fh_device_free": ([_VP, _VP], ctypes.c_int),
     "fh_copy_to_device": ([_VP, _VP, _VP, ctypes.c_int64], ctypes.c_int),

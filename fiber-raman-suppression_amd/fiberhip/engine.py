@@ -98,7 +98,8 @@ class Plan:
     """
 
     def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: int,
-                 precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None):
+                 precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None,
+                 band_mask=None, checkpoint_every: int = 50, log_cost: bool = False):
         self.context = context or default_context()
         D = np.asarray(dispersion, dtype=np.float64)
         if D.ndim == 1:
@@ -113,23 +114,29 @@ class Plan:
         self._g = np.ascontiguousarray(g.transpose(3, 2, 1, 0))
         self._hr = np.ascontiguousarray(np.asarray(raman, dtype=np.complex128).reshape(-1))
         self._sh = np.ascontiguousarray(np.asarray(shock, dtype=np.float64).reshape(-1))
+        self._mask = None if band_mask is None else np.ascontiguousarray(np.asarray(band_mask, dtype=np.uint8))
+        self.has_band = band_mask is not None
         desc = _lib.ProblemDesc(
             nt=self.nt, modes=self.modes, precision=PRECISIONS[precision], scheme=FH_SCHEME_RK4IP_FIXED,
             steps=self.steps, max_batch=self.max_batch, length_m=self.length_m,
             dispersion=_dp(self._D), gamma=_dp(self._g), raman=_dp(self._hr.view(np.float64)),
-            shock=_dp(self._sh), one_m_fr=float(one_m_fr))
+            shock=_dp(self._sh), one_m_fr=float(one_m_fr), checkpoint_every=int(checkpoint_every),
+            log_cost=int(bool(log_cost)),
+            band_mask=None if self._mask is None else self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
         h = ctypes.c_void_p()
         check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
 
     @classmethod
     def from_problem(cls, problem, steps: int, precision: str = "fp64", max_batch: int = 1,
-                     context: Optional[Context] = None) -> "Plan":
+                     context: Optional[Context] = None, band_mask=None, checkpoint_every: int = 50,
+                     log_cost: bool = False) -> "Plan":
         f = problem.fiber
         from .problem import self_steepening
 
         return cls(f["Dω"], f["γ"], f["hRω"], self_steepening(problem.sim), f["one_m_fR"], f["L"], steps,
-                   precision=precision, max_batch=max_batch, context=context)
+                   precision=precision, max_batch=max_batch, context=context, band_mask=band_mask,
+                   checkpoint_every=checkpoint_every, log_cost=log_cost)
 
     @property
     def device_bytes(self) -> int:
@@ -156,6 +163,47 @@ class Plan:
         stats = dict(accepted=st.accepted, rejected=st.rejected, rhs_evaluations=st.rhs_evals,
                      retcode=st.retcode)
         return (res[0] if single else res), stats
+
+    def gradient(self, u0, seed=None) -> dict:
+        """Discrete-adjoint gradient (fh_gradient).  u0: (nt, m) or (batch, nt, m)
+        shaped launches; seed: matching Wirtinger terminal seeds, or None for
+        the plan's band objective.  Returns uL, lam0, grad (per bin), cost."""
+        u = np.asarray(u0, dtype=np.complex128)
+        single = u.ndim == 2
+        if single:
+            u = u[None]
+        if u.ndim != 3 or u.shape[1:] != (self.nt, self.modes):
+            raise ValueError(f"launch has shape {u.shape}, expected (batch, {self.nt}, {self.modes})")
+        batch = u.shape[0]
+        buf = np.ascontiguousarray(np.transpose(u, (0, 2, 1)))
+        sd = None
+        if seed is not None:
+            sv = np.asarray(seed, dtype=np.complex128)
+            sv = sv[None] if sv.ndim == 2 else sv
+            if sv.shape != u.shape:
+                raise ValueError("seed must match the launch shape")
+            sd = np.ascontiguousarray(np.transpose(sv, (0, 2, 1)))
+        uL = np.empty_like(buf)
+        lam0 = np.empty_like(buf)
+        grad = np.empty((batch, self.nt))
+        cost = np.empty(batch)
+        st = _lib.Stats()
+        check(lib().fh_gradient(self.handle, batch, _dp(buf.view(np.float64)),
+                                None if sd is None else _dp(sd.view(np.float64)), _dp(uL.view(np.float64)),
+                                _dp(lam0.view(np.float64)), _dp(grad), _dp(cost), ctypes.byref(st)))
+        out = dict(uL=np.transpose(uL, (0, 2, 1)), lam0=np.transpose(lam0, (0, 2, 1)), grad=grad, cost=cost,
+                   rhs_evaluations=st.rhs_evals)
+        if single:
+            out = {k: (v[0] if isinstance(v, np.ndarray) else v) for k, v in out.items()}
+        return out
+
+    def time_gradient_device(self, batch: int, d_u0: int, d_uL: int, d_lam0: int, d_grad: int, d_cost: int,
+                             launches: int) -> np.ndarray:
+        ms = (ctypes.c_float * launches)()
+        check(lib().fh_time_gradient_device(self.handle, int(batch), ctypes.c_void_p(d_u0), ctypes.c_void_p(d_uL),
+                                            ctypes.c_void_p(d_lam0), ctypes.c_void_p(d_grad),
+                                            ctypes.c_void_p(d_cost), int(launches), ms))
+        return np.array(ms[:], dtype=np.float64)
 
     def forward_device(self, batch: int, d_u0: int, d_uL: int, stream: int = 0) -> None:
         """Enqueue a forward solve on device-resident fields (pointers as ints)."""

@@ -78,6 +78,9 @@ typedef struct {
     const double *raman;      /* fiber["hRw"], nt complex (2*nt doubles)    */
     const double *shock;      /* fftshift(sim["ws"]/sim["w0"]), nt          */
     double one_m_fr;          /* fiber["one_m_fR"]                          */
+    int32_t checkpoint_every; /* adjoint: forward states kept every K steps (0 = 50) */
+    int32_t log_cost;         /* band objective in dB (_maybe_log_cost)      */
+    const uint8_t *band_mask; /* nt bins of the band objective, or NULL      */
 } fh_problem_desc;
 
 typedef struct {
@@ -112,6 +115,34 @@ int fh_forward(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_sta
  * for FH_PREC_FP64, complex64 for FH_PREC_FP32_BULK), enqueued on `stream`
  * (a hipStream_t; NULL = the context stream).  Does not synchronise. */
 int fh_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *stream);
+
+/* Gradient of a terminal objective through the discrete adjoint of the
+ * fixed-step RK4IP map (the role of solve_adjoint_disp_mmf plus the chain
+ * rule of spectral_shaper_model, sensitivity_disp_mmf.jl:309-332,
+ * physics_models.jl:1042-1046).  For each of `batch` launches u0 (host,
+ * complex128, the already-shaped spectrum alpha*A*cis(phi)*uw0):
+ *   seed != NULL : lamL = seed, the caller's Wirtinger terminal adjoint
+ *                  dJ/dconj(uL) (terminal_adjoint, adjoints.jl:723-739);
+ *   seed == NULL : the band objective J = E_band/E_total of the plan's
+ *                  band_mask and its seed uL (mask - J)/E_total
+ *                  (_field_spectral_band_cost, physics_helpers.jl:222-232),
+ *                  in dB when the plan's log_cost is set.
+ * Outputs (any may be NULL): uL (lab-frame output), lam0 = dJ/dconj(u0),
+ * grad = 2 Re(conj(lam0) i u0) per bin summed over modes (the FullGridPhase
+ * gradient, accumulated in FP64), cost (J per launch; NaN with a seed).
+ * Forward states are checkpointed every checkpoint_every steps in device
+ * memory and each segment is recomputed once during the backward sweep. */
+int fh_gradient(fh_plan plan, int32_t batch, const double *u0, const double *seed, double *uL, double *lam0,
+                double *grad, double *cost, fh_stats *stats);
+
+/* Same on device-resident arrays in the plan's precision (grad and cost are
+ * double); enqueued on `stream` (NULL = context stream), no synchronisation. */
+int fh_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, const void *d_seed, void *d_uL,
+                       void *d_lam0, double *d_grad, double *d_cost, void *stream);
+
+/* Event-timed back-to-back gradient launches (see fh_time_forward_device). */
+int fh_time_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *d_lam0,
+                            double *d_grad, double *d_cost, int32_t launches, float *ms_out);
 
 /* FP64 per-mode spectral energy E_m = sum_w |u(w,m)|^2 of `batch` device
  * fields in the plan's precision, written to host energies[batch*modes]

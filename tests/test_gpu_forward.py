@@ -149,8 +149,8 @@ def test_energy_device_matches_host(gpu_context):
 
     problem = smf_problem(nt=1024, window_ps=10.0, length_m=0.1, policy="exact")
     for precision in ("fp64", "fp32"):
-        plan = fiberhip.Plan.from_problem(problem, steps=10, precision=precision, max_batch=4)
-        host = np.stack([problem.uω0 * (i + 1) for i in range(3)])
+        plan = fiberhip.Plan.from_problem(problem, steps=40, precision=precision, max_batch=4)
+        host = np.stack([problem.uω0 * (1.0 - 0.2 * i) for i in range(3)])
         dev = plan.to_device(host)
         e = plan.energy_device(3, dev.ptr)
         cast = host.astype(plan.field_dtype).astype(np.complex128)
