@@ -38,6 +38,17 @@ def forward_flops(nt: int, m: int, steps: int) -> float:
     return steps * (4 * rhs_flops(nt, m) + 28 * m * nt)
 
 
+def adjoint_rhs_flops(nt: int, m: int) -> float:
+    """Algorithmic flops of one adjoint_disp_mmf! evaluation (8M + 6M^2 FFTs,
+    7 gamma contractions, pointwise work)."""
+    return 5 * nt * math.log2(nt) * (8 * m + 6 * m * m) + 14 * m ** 4 * nt + 60 * m * m * nt + 40 * m * nt
+
+
+def gradient_flops(nt: int, m: int, steps: int) -> float:
+    """Forward solve + adjoint sweep + chain rule; checkpoint recomputation not counted."""
+    return forward_flops(nt, m, steps) + steps * (4 * adjoint_rhs_flops(nt, m) + 28 * m * nt) + 8 * m * nt
+
+
 def workload(name: str):
     import fiberhip
 
@@ -46,6 +57,11 @@ def workload(name: str):
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=8192, time_window_ps=10.0),
                                          raman_threshold_thz=None)
         return problem, 200
+    if name == "config2":
+        fib = fiberhip.Fiber(length_m=2.0, power_w=0.2, preset="SMF28", beta_order=3)
+        problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=16384, time_window_ps=47.061, policy="exact"),
+                                         raman_threshold_thz=None)
+        return problem, 2000
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -77,6 +93,64 @@ def time_path(problem, steps, precision, batch, k, warmup):
     energies = plan.energy_device(batch, out.ptr)
     return dict(plan=plan, wall_s=wall, kernel_ms=float(np.mean(ms)), kernel_ms_p50=float(np.median(ms)),
                 energies=energies)
+
+
+def time_gradient(problem, steps, precision, batch, k, warmup, mask):
+    """K event-timed launches of the fused gradient kernel (band objective on device)."""
+    import fiberhip
+
+    plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch, band_mask=mask,
+                                      checkpoint_every=50)
+    u0 = plan.to_device(launches(problem, batch, seed=2)[:, :, None])
+    out = fiberhip.DeviceArray(plan.context, u0.nbytes)
+    lam0 = fiberhip.DeviceArray(plan.context, u0.nbytes)
+    grad = fiberhip.DeviceArray(plan.context, 8 * batch * problem.nt)
+    cost = fiberhip.DeviceArray(plan.context, 8 * batch)
+    for _ in range(warmup):
+        plan.time_gradient_device(batch, u0.ptr, out.ptr, lam0.ptr, grad.ptr, cost.ptr, 1)
+    print(f"bench: {precision} gradient warmup done, timing {k} launches of {batch}", file=sys.stderr, flush=True)
+    ms = plan.time_gradient_device(batch, u0.ptr, out.ptr, lam0.ptr, grad.ptr, cost.ptr, k)
+    energies = plan.energy_device(batch, out.ptr)
+    return dict(kernel_ms=float(np.mean(ms)), ms=ms, energies=energies,
+                grad=grad.to_host(np.float64, (batch, problem.nt)), cost=cost.to_host(np.float64, (batch,)),
+                device_bytes=plan.device_bytes)
+
+
+def cpu_baseline_gradient(problem, steps, mask, slice_steps=200):
+    """The oracle's C discrete adjoint (FP64, one thread per gradient) on the
+    host cores.  Bounded sample: every thread runs one gradient over the first
+    `slice_steps` of the workload's RK4 steps (same grid, step size and
+    checkpoint spacing); the per-step cost is uniform, so the full-length
+    rate is the sample rate times slice_steps/steps."""
+    import fiberhip
+    import oracle
+    from oracle import np_ref
+
+    f = problem.fiber
+    fib = oracle.Fiber(f["Dω"], f["γ"], f["hRω"], fiberhip.self_steepening(problem.sim), f["one_m_fR"])
+    u = launches(problem, 16, seed=2)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    h = f["L"] / steps
+    length = h * slice_steps
+
+    def work(i):
+        u0 = u[i % 16][:, None]
+        uL = oracle.rk4_forward(fib, length, slice_steps, u0)
+        _, seed = np_ref.band_cost(uL, mask)
+        oracle.rk4_adjoint(fib, length, slice_steps, u0, seed, every=50)
+
+    t = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t
+    value = threads / wall * slice_steps / steps
+    return dict(value=value, unit="grads/s", cores=threads, kind="port",
+                sample=f"{threads} config-2 gradients (forward + discrete adjoint, checkpoint every 50) over the first "
+                       f"{slice_steps} of {steps} RK4 steps at Nt={problem.nt}, one per thread, {wall:.1f} s wall; "
+                       f"rate scaled by {slice_steps}/{steps}")
 
 
 def cpu_baseline(problem, steps, budget_s=20.0):
@@ -119,11 +193,12 @@ def cpu_baseline(problem, steps, budget_s=20.0):
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=1024)
-    ap.add_argument("--workload", default="config0")
+    ap.add_argument("--steps", type=int, default=3, help="timed launches per path")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256, help="gradients per launch (config 2)")
+    ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (config 0)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["all", "gradient", "forward"], default="all")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -143,49 +218,82 @@ def main() -> None:
         print("bench.py: this build measures one GPU only (independent solves shard across GPUs without "
               "communication; see DESIGN.md); run with --gpus 1", file=sys.stderr)
         raise SystemExit(2)
-    problem, steps = workload(args.workload)
-    nt, m = problem.nt, problem.modes
-    res = {}
-    for precision in ("fp32", "fp64"):
-        res[precision] = time_path(problem, steps, precision, args.batch, args.steps, args.warmup)
-    e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
-    err = float(np.max(np.abs(e32 - e64) / e64))
-    flops = forward_flops(nt, m, steps)
-    head = res["fp32"]
-    ms_step = head["kernel_ms"]
-    value = args.batch / (ms_step / 1e3)
-    fp64_value = args.batch / (res["fp64"]["kernel_ms"] / 1e3)
-    achieved = args.batch * flops / (ms_step / 1e3) / 1e12
-    line = {
-        "metric": "fwd solves/sec (FP32-bulk batched path) vs plain FP64 GPU @<=1e-5 err",
-        "value": value,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: SMF-28 0.5 m, 0.1 W, 185 fs sech2, Nt={nt}, "
-                               f"T={problem.sim['time_window']} ps, {steps} RK4IP steps",
-                   "batch_solves_per_step": args.batch, "modes": m, "nt": nt},
-        "fp64_value": fp64_value,
-        "fp32_over_fp64": value / fp64_value,
-        "fp32_vs_fp64_max_rel_energy_err": err,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
-                     "kernel": "k_rk4ip_single<float,8192,512>",
-                     "flops_per_solve": flops},
-        "fp64_roofline": {"achieved": args.batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12,
-                          "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-                          "kernel": "k_rk4ip_single<double,8192,1024>"},
-    }
-    if not args.no_cpu:
-        cb = cpu_baseline(problem, steps)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    import fiberhip
+
+    line = {}
+    if args.only in ("all", "gradient"):
+        problem, steps = workload("config2")
+        nt, m = problem.nt, problem.modes
+        mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+        g = {p: time_gradient(problem, steps, p, args.batch, args.steps, args.warmup, mask) for p in ("fp32", "fp64")}
+        ms = g["fp32"]["kernel_ms"]
+        value = args.batch / (ms / 1e3)
+        fp64_value = args.batch / (g["fp64"]["kernel_ms"] / 1e3)
+        flops = gradient_flops(nt, m, steps)
+        achieved = args.batch * flops / (ms / 1e3) / 1e12
+        e32, e64 = g["fp32"]["energies"], g["fp64"]["energies"]
+        gerr = np.linalg.norm(g["fp32"]["grad"] - g["fp64"]["grad"], axis=1) / np.linalg.norm(g["fp64"]["grad"], axis=1)
+        line = {
+            "metric": "adjoint grads/sec (FP32-bulk batched path) vs plain FP64 GPU @<=1e-5 err",
+            "value": value,
+            "unit": "grads/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"config2: SMF-28 2 m, 0.2 W, 185 fs sech2, Nt={nt}, T={problem.sim['time_window']} ps, "
+                                   f"{steps} RK4IP steps, checkpoint every 50, band -16..-10 THz objective, "
+                                   f"gradient w.r.t. the full-grid spectral phase",
+                       "batch_gradients_per_step": args.batch, "modes": m, "nt": nt},
+            "fp64_value": fp64_value,
+            "fp32_over_fp64": value / fp64_value,
+            "fp32_vs_fp64": {"max_rel_output_spectral_energy_err": float(np.max(np.abs(e32 - e64) / e64)),
+                             "max_rel_cost_err": float(np.max(np.abs(g["fp32"]["cost"] - g["fp64"]["cost"])
+                                                              / np.abs(g["fp64"]["cost"]))),
+                             "max_rel_gradient_err_normwise": float(np.max(gerr))},
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+                         "kernel": "k_grad_single<float,16384,1024>", "flops_per_gradient": flops},
+            "fp64_roofline": {"achieved": args.batch * flops / (g["fp64"]["kernel_ms"] / 1e3) / 1e12,
+                              "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "kernel": "k_grad_single<double,16384,1024>"},
+            "device_bytes_fp32_plan": g["fp32"]["device_bytes"],
+        }
+        if not args.no_cpu:
+            cb = cpu_baseline_gradient(problem, steps, mask)
+            line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    if args.only in ("all", "forward"):
+        problem, steps = workload("config0")
+        nt, m = problem.nt, problem.modes
+        res = {p: time_path(problem, steps, p, args.fwd_batch, args.steps, args.warmup) for p in ("fp32", "fp64")}
+        e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
+        flops = forward_flops(nt, m, steps)
+        fwd = {
+            "workload": f"config0: SMF-28 0.5 m, 0.1 W, Nt={nt}, T={problem.sim['time_window']} ps, {steps} RK4IP steps",
+            "batch_solves_per_step": args.fwd_batch,
+            "fp32_solves_per_s": args.fwd_batch / (res["fp32"]["kernel_ms"] / 1e3),
+            "fp64_solves_per_s": args.fwd_batch / (res["fp64"]["kernel_ms"] / 1e3),
+            "fp32_ms_per_launch": res["fp32"]["kernel_ms"],
+            "fp64_ms_per_launch": res["fp64"]["kernel_ms"],
+            "fp32_vs_fp64_max_rel_energy_err": float(np.max(np.abs(e32 - e64) / e64)),
+            "fp32_roofline_frac": args.fwd_batch * flops / (res["fp32"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp32"],
+            "fp64_roofline_frac": args.fwd_batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp64"],
+        }
+        if not args.no_cpu and args.only == "forward":
+            cb = cpu_baseline(problem, steps)
+            fwd["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        if args.only == "forward":
+            line = {"metric": "fwd solves/sec (FP32-bulk batched path)", "value": fwd["fp32_solves_per_s"],
+                    "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                    "ms_per_step": fwd["fp32_ms_per_launch"], "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                    "config": {"workload": fwd["workload"]}, "forward": fwd}
+        else:
+            line["forward_config0"] = fwd
     print(json.dumps(line), flush=True)
 
 
