@@ -1,0 +1,22 @@
+// fh_launch.hpp -- launch shapes and cross-unit launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fh {
+
+template <typename R> struct MmArgs;
+
+// Workgroup size per precision and transform length: 8 samples per thread
+// for FP64 (1024 threads from 8192 samples up) keeps the radix-8 working set
+// in registers; FP32 uses 16 from 4096 samples up.
+template <typename R, int N> struct SingleShape {
+    static constexpr int T = sizeof(R) == 8 ? (N >= 8192 ? 1024 : (N >= 256 ? N / 8 : N / 4))
+                                            : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4));
+};
+
+bool multi_supported(int nt, int m, bool fp64);
+int multi_fields(int m);
+hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s);
+
+}  // namespace fh

@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "../../include/fiberhip.h"
+#include "fh_launch.hpp"
+#include "fh_multi.hpp"
 #include "fh_single.hpp"
 
 namespace {
@@ -50,6 +52,8 @@ struct fh_plan_s {
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
     int every = 50, nseg = 0;
     bool has_mask = false;
+    DevBuf gr, qidx;  // multimode: reduced gamma [Q][P] and output-pair index [M][M]
+    int q = 0;
     size_t work_per_solve = 0;  // complex entries
     int64_t total_bytes = 0;
     double gamma0 = 0.0;        // gamma[1,1,1,1] of a single-mode plan
@@ -98,13 +102,7 @@ template <int N, int T> void fill_twiddles(std::vector<std::complex<long double>
 
 // ------------------------------------------------------------ dispatch
 
-// Workgroup size of the single-mode kernel per precision and length: 8
-// samples per thread for FP64 (1024 threads from 8192 samples up) keeps the
-// radix-8 working set in registers; FP32 uses 16 from 4096 samples up.
-template <typename R, int N> struct SingleShape {
-    static constexpr int T = sizeof(R) == 8 ? (N >= 8192 ? 1024 : (N >= 256 ? N / 8 : N / 4))
-                                            : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4));
-};
+using fh::SingleShape;
 
 template <typename R, int N> struct SingleKernel {
     static constexpr int T = SingleShape<R, N>::T;
@@ -227,8 +225,12 @@ int check_desc(const fh_problem_desc* d)
     if (dmax * d->length_m >= FH_MAX_PHASE)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
-    if (d->modes != 1) return fail(FH_EUNSUPPORTED, "this build integrates single-mode problems only");
-    if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
+    if (d->modes == 1) {
+        if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
+    } else if (!fh::multi_supported(d->nt, d->modes, d->precision == FH_PREC_FP64)) {
+        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {2,3,4,6,10} and nt in [256, 8192] (FP64) or "
+                                     "[256, 16384] (FP32)");
+    }
     return FH_OK;
 }
 
@@ -271,7 +273,53 @@ template <typename R> int upload_tables(fh_plan p)
     p->every = d.checkpoint_every > 0 ? d.checkpoint_every : 50;
     if (p->every > d.steps) p->every = d.steps;
     p->nseg = (d.steps + p->every - 1) / p->every;
-    p->work_per_solve = with_size<R, OpWork>(nt);
+    p->work_per_solve = d.modes == 1 ? with_size<R, OpWork>(nt) : (size_t)fh::multi_fields(d.modes) * nt;
+    if (d.modes > 1) {
+        const int M = d.modes;
+        auto G = [&](int i, int j, int k, int l) { return d.gamma[i + M * (j + M * (k + M * l))]; };
+        const double gmax = [&] {
+            double g = 0;
+            for (int i = 0; i < M * M * M * M; ++i) g = std::max(g, std::fabs(d.gamma[i]));
+            return g;
+        }();
+        bool sym = true;
+        for (int i = 0; i < M && sym; ++i)
+            for (int j = 0; j < M && sym; ++j)
+                for (int k = 0; k < M && sym; ++k)
+                    for (int l = 0; l < M; ++l)
+                        if (std::fabs(G(i, j, k, l) - G(j, i, k, l)) > 1e-12 * std::max(1.0, gmax)) {
+                            sym = false;
+                            break;
+                        }
+        std::vector<std::pair<int, int>> outp, inp;
+        for (int i = 0; i < M; ++i)
+            for (int j = i; j < M; ++j) inp.push_back({i, j});
+        std::vector<int> qidx(M * M);
+        if (sym) {
+            for (int i = 0; i < M; ++i)
+                for (int j = i; j < M; ++j) {
+                    qidx[i * M + j] = qidx[j * M + i] = (int)outp.size();
+                    outp.push_back({i, j});
+                }
+        } else {
+            for (int i = 0; i < M; ++i)
+                for (int j = 0; j < M; ++j) {
+                    qidx[i * M + j] = (int)outp.size();
+                    outp.push_back({i, j});
+                }
+        }
+        p->q = (int)outp.size();
+        std::vector<R> gr(outp.size() * inp.size());
+        for (size_t qq = 0; qq < outp.size(); ++qq)
+            for (size_t pp = 0; pp < inp.size(); ++pp) {
+                const int i = outp[qq].first, j = outp[qq].second, k = inp[pp].first, l = inp[pp].second;
+                gr[qq * inp.size() + pp] = (R)(k == l ? G(i, j, k, k) : G(i, j, k, l) + G(i, j, l, k));
+            }
+        if ((rc = dev_alloc(p, p->gr, sizeof(R) * gr.size()))) return rc;
+        if ((rc = dev_alloc(p, p->qidx, sizeof(int) * qidx.size()))) return rc;
+        FH_HIP(hipMemcpy(p->gr.p, gr.data(), p->gr.bytes, hipMemcpyHostToDevice));
+        FH_HIP(hipMemcpy(p->qidx.p, qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice));
+    }
     const size_t mb = (size_t)d.max_batch;
     if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
     if ((rc = dev_alloc(p, p->status, sizeof(int) * mb))) return rc;
@@ -322,6 +370,7 @@ template <typename R>
 int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
                     double* d_grad, double* d_cost, hipStream_t s)
 {
+    if (p->d.modes != 1) return fail(FH_EUNSUPPORTED, "the adjoint kernel of this build is single-mode");
     if (!d_seed && !p->has_mask)
         return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
     int rc = ensure_adjoint_buffers<R>(p);
@@ -342,11 +391,36 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
     return FH_OK;
 }
 
+template <typename R> int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+{
+    const fh_problem_desc& d = p->d;
+    fh::MmArgs<R> a{};
+    a.u0 = (const fh::C2<R>*)d_u0;
+    a.out = (fh::C2<R>*)d_uL;
+    a.work = (fh::C2<R>*)p->work.p;
+    a.disp = (const double*)p->disp.p;
+    a.hn = (const fh::C2<R>*)p->hn.p;
+    a.shn = (const R*)p->shn.p;
+    a.tw = (const fh::C2<R>*)p->tw.p;
+    a.gr = (const R*)p->gr.p;
+    a.qidx = (const int*)p->qidx.p;
+    a.status = (int*)p->status.p;
+    a.one_m_fr = (R)d.one_m_fr;
+    a.h = d.length_m / d.steps;
+    a.length = d.length_m;
+    a.steps = d.steps;
+    a.q = p->q;
+    FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    hipError_t e = fh::launch_multi(d.nt, d.modes, a, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode launch: ") + hipGetErrorString(e));
+    return FH_OK;
+}
+
 template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
 {
     const fh_problem_desc& d = p->d;
+    if (d.modes > 1) return launch_forward_multi<R>(p, batch, d_u0, d_uL, s);
     fh::SmArgs<R> a = base_args<R>(p);
-    (void)d;
     a.u0 = (const fh::C2<R>*)d_u0;
     a.out = (fh::C2<R>*)d_uL;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
@@ -414,7 +488,7 @@ int fh_plan_destroy(fh_plan p)
     if (!p) return FH_OK;
     (void)hipSetDevice(p->ctx->device);
     for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out, &p->mask,
-                      &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed})
+                      &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed, &p->gr, &p->qidx})
         dev_free(*b);
     delete p;
     return FH_OK;

@@ -1,0 +1,135 @@
+"""Parabolic GRIN multimode fibers with analytic Hermite-Gaussian modes.
+
+Builds the operators that get_disp_fiber_params / get_params produce
+(fibers.jl:91-132): the Kerr tensor gamma = S_K n2 w0 / c from the overlap
+integral of unit-norm sampled modes (compute_overlap_tensor, fibers.jl:214-222)
+and the dispersion D(w) from finite differences of beta(f) on a symmetric
+stencil, referenced to mode 1 (fibers.jl:98-118).
+
+The reference finds its modes with a 5-point finite-difference eigensolve
+(Arpack, fibers.jl:159-197).  Here the modes of the parabolic profile are the
+harmonic-oscillator (Hermite-Gaussian) solutions of the same scalar wave
+equation with eps(r) ~ n_co^2 - g^2 r^2, g^2 = 2 n_co (n_co - n_cl) / a^2:
+    psi_pq ~ H_p(x/s) H_q(y/s) exp(-(x^2+y^2)/(2 s^2)),  s^2 = 1/(k g)
+    beta_pq^2 = k^2 n_co^2 - 2 k g (p + q + 1).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .problem import (
+    FiberFieldProblem,
+    fftfreq,
+    fiber_field_problem,
+    get_disp_sim_params,
+    get_initial_state,
+    raman_fields,
+)
+
+N2_SILICA = 2.3e-20  # m^2/W (fibers.jl:125)
+MODE_ORDER = [(0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3)]
+# 5-point central finite-difference weights for the first and second derivative
+FD5 = {1: np.array([1 / 12, -2 / 3, 0.0, 2 / 3, -1 / 12]),
+       2: np.array([-1 / 12, 4 / 3, -5 / 2, 4 / 3, -1 / 12])}
+
+
+def silica_index(wavelength_um: float) -> float:
+    """Sellmeier index of fused silica (coefficients of build_GRIN, fibers.jl:30-38)."""
+    l2 = wavelength_um ** 2
+    return math.sqrt(1 + 0.6961663 * l2 / (l2 - 0.0684043 ** 2) + 0.4079426 * l2 / (l2 - 0.1162414 ** 2)
+                     + 0.8974794 * l2 / (l2 - 9.896161 ** 2))
+
+
+def _profile(wavelength_um: float, radius_um: float, core_na: float):
+    nco = silica_index(wavelength_um)
+    ncl = math.sqrt(nco ** 2 - core_na ** 2)
+    k = 2 * math.pi / wavelength_um
+    g = math.sqrt(2 * nco * (nco - ncl)) / radius_um
+    return nco, k, g
+
+
+def hermite(n: int, x: np.ndarray) -> np.ndarray:
+    """Physicists' Hermite polynomial by recurrence."""
+    h0, h1 = np.ones_like(x), 2 * x
+    if n == 0:
+        return h0
+    for k in range(1, n):
+        h0, h1 = h1, 2 * x * h1 - 2 * k * h0
+    return h1
+
+
+def hg_modes(modes: int, wavelength_um=1.55, radius_um=25.0, core_na=0.2, nx=256, window_um=80.0):
+    """Unit-norm sampled modes (nx, nx, M) on the grid x = (-nx/2 .. nx/2-1) dx (fibers.jl:45-47)."""
+    _, k, g = _profile(wavelength_um, radius_um, core_na)
+    s = 1 / math.sqrt(k * g)
+    dx = window_um / nx
+    x = np.arange(-nx // 2, nx // 2) * dx
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    out = np.empty((nx, nx, modes))
+    for i, (p, q) in enumerate(MODE_ORDER[:modes]):
+        psi = hermite(p, X / s) * hermite(q, Y / s) * np.exp(-(X ** 2 + Y ** 2) / (2 * s * s))
+        out[:, :, i] = psi / math.sqrt(np.sum(psi ** 2))
+    return out, x, dx
+
+
+def overlap_tensor(modes: np.ndarray, dx_si: float) -> np.ndarray:
+    """S_K[i,j,k,l] = sum_mn phi_i phi_j phi_k phi_l / dx^2 (fibers.jl:214-222)."""
+    nx = modes.shape[0]
+    f = modes.reshape(nx * nx, -1)
+    m = f.shape[1]
+    pair = np.einsum("ri,rj->rij", f, f).reshape(nx * nx, m * m)
+    return (pair.T @ pair).reshape(m, m, m, m) / dx_si ** 2
+
+
+def beta_pq(f_thz: float, p: int, q: int, radius_um=25.0, core_na=0.2) -> float:
+    """Propagation constant [1/m] of HG mode (p, q) at frequency f."""
+    lam_um = 2.99792458e8 / (f_thz * 1e12) * 1e6
+    nco, k, g = _profile(lam_um, radius_um, core_na)
+    return math.sqrt(k * k * nco * nco - 2 * k * g * (p + q + 1)) * 1e6
+
+
+def grin_dispersion(sim: dict, modes: int, beta_order=2, delta_f_thz=1.0, radius_um=25.0, core_na=0.2):
+    """D(w) (nt, M): beta(f) on a 2*beta_order+1 point stencil, central finite
+    differences, beta0/beta1 referenced to mode 1 (fibers.jl:92-118)."""
+    if beta_order != 2:
+        raise ValueError("the analytic GRIN builder uses beta_order = 2 (5-point stencil)")
+    f0 = sim["f0"]
+    offsets = np.arange(-2, 3)
+    bf = np.array([[beta_pq(f0 + delta_f_thz * o, p, q, radius_um, core_na) for (p, q) in MODE_ORDER[:modes]]
+                   for o in offsets])  # (5, M)
+    d = {n: FD5[n] @ bf / (2 * math.pi * delta_f_thz / 1e-12) ** n for n in (1, 2)}
+    betas = np.stack([bf[2] - bf[2, 0], d[1] - d[1][0], d[2]], axis=0)  # (3, M)
+    w = 2 * math.pi * fftfreq(int(sim["Nt"]), 1 / sim["Δt"]) * 1e12
+    cols = np.stack([w ** n / math.factorial(n) for n in range(3)], axis=1)
+    return cols @ betas
+
+
+def grin_problem(modes: int = 6, nt: int = 4096, time_window_ps: float = 12.0, length_m: float = 0.3,
+                 energy_j: float = 10e-9, fwhm_s: float = 100e-15, seed: int = 1, nx: int = 256,
+                 window_um: float = 80.0, radius_um=25.0, core_na=0.2, wavelength_m=1550e-9, fR=0.18,
+                 launch=None) -> FiberFieldProblem:
+    """Config-1 problem: parabolic GRIN (50 um core, NA 0.2), M HG modes on an
+    nx^2 grid, sech^2 pulse of energy_j split over modes by Dirichlet(1)
+    weights with uniform phases (seeded), initial_modes normalised
+    (physics_models.jl:289-297)."""
+    sim = get_disp_sim_params(wavelength_m, modes, nt, time_window_ps, 2)
+    phi, _, dx = hg_modes(modes, wavelength_m * 1e6, radius_um, core_na, nx, window_um)
+    sk = overlap_tensor(phi, dx * 1e-6)
+    w0 = 2 * math.pi * sim["f0"] * 1e12
+    gamma = sk * N2_SILICA * w0 / 2.99792458e8
+    D = grin_dispersion(sim, modes, radius_um=radius_um, core_na=core_na)
+    if launch is None:
+        rng = np.random.default_rng(seed)
+        w = rng.dirichlet(np.ones(modes))
+        theta = rng.uniform(0, 2 * math.pi, modes)
+        launch = np.sqrt(w) * np.exp(1j * theta)
+    launch = np.asarray(launch, complex)
+    launch = launch / np.linalg.norm(launch)
+    rep = 1.0  # energy per pulse = P_avg / rep_rate
+    _, uw0 = get_initial_state(launch, energy_j * rep, fwhm_s, rep, "sech_sq", sim)
+    fiber = {"ϕ": phi, "Dω": D, "γ": gamma, "L": float(length_m), "zsave": None, "x": None,
+             "gain_parameters": 0.0}
+    fiber.update(raman_fields(sim, fR))
+    return fiber_field_problem(uw0, fiber, sim, preset="GRIN_50_HG")

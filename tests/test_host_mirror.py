@@ -4,6 +4,7 @@ restatement of the same helpers, error behaviour, and the C ABI library
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
 import re
 import subprocess
@@ -119,3 +120,47 @@ def test_last_error_starts_empty_and_bad_args_fail_without_gpu():
     assert lib.fh_context_create(0, None) == _lib.FH_EINVAL
     assert b"NULL" in lib.fh_last_error()
     assert lib.fh_plan_create(None, None, None) == _lib.FH_EINVAL
+
+
+def test_grin_builder_invariants():
+    """Analytic HG modes: unit norm, orthogonal; S_K fully symmetric and
+    S_0000 = 1/(2 pi s^2) for the Gaussian fundamental; D referenced to
+    mode 1 (beta0 = beta1 = 0 for mode 1) and equal for degenerate modes."""
+    import math
+
+    from fiberhip import grin
+
+    phi, x, dx = grin.hg_modes(6)
+    gram = np.einsum("abi,abj->ij", phi, phi)
+    assert np.allclose(gram, np.eye(6), atol=1e-10)
+    sk = grin.overlap_tensor(phi, dx * 1e-6)
+    for perm in itertools.permutations(range(4)):
+        assert np.allclose(sk, np.transpose(sk, perm), rtol=1e-12, atol=1e-12 * np.abs(sk).max())
+    _, k, g = grin._profile(1.55, 25.0, 0.2)
+    s2 = 1 / (k * g) * 1e-12
+    assert math.isclose(sk[0, 0, 0, 0], 1 / (2 * math.pi * s2), rel_tol=1e-9)
+    # independent restatement of compute_overlap_tensor on a coarse grid
+    small = phi[::8, ::8, :3]
+    ref = np.zeros((3,) * 4)
+    for i, j, kk, l in itertools.product(range(3), repeat=4):
+        ref[i, j, kk, l] = np.sum(small[:, :, i] * small[:, :, j] * small[:, :, kk] * small[:, :, l])
+    assert np.allclose(grin.overlap_tensor(small, 1.0), ref, rtol=1e-12, atol=1e-15)
+    sim = fiberhip.get_disp_sim_params(1550e-9, 6, 1024, 10.0, 2)
+    D = grin.grin_dispersion(sim, 6)
+    assert D[0, 0] == 0.0
+    assert np.allclose(D[:, 1], D[:, 2]) and np.allclose(D[:, 3], D[:, 5])
+    w = 2 * np.pi * fiberhip.fftfreq(1024, 1 / sim["Δt"]) * 1e12
+    # mode 1 keeps only beta2 w^2 / 2
+    beta2 = D[1, 0] / (w[1] ** 2 / 2)
+    assert np.allclose(D[:, 0], beta2 * w ** 2 / 2, rtol=1e-9, atol=1e-12)
+
+
+def test_config1_launch():
+    """10 nJ sech2 100 fs -> P_peak = 0.881374 E / fwhm = 88.137 kW, split by
+    seeded Dirichlet weights."""
+    from fiberhip import grin
+
+    p = grin.grin_problem(6)
+    ut = np.fft.fft(p.uω0, axis=0)
+    assert math.isclose(np.max(np.sum(np.abs(ut) ** 2, axis=1)), 0.881374 * 10e-9 / 100e-15, rel_tol=1e-12)
+    assert p.uω0.shape == (4096, 6)
