@@ -57,6 +57,10 @@ def workload(name: str):
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=8192, time_window_ps=10.0),
                                          raman_threshold_thz=None)
         return problem, 200
+    if name == "config1":
+        from fiberhip import grin
+
+        return grin.grin_problem(6), 4000
     if name == "config2":
         fib = fiberhip.Fiber(length_m=2.0, power_w=0.2, preset="SMF28", beta_order=3)
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=16384, time_window_ps=47.061, policy="exact"),
@@ -76,12 +80,12 @@ def launches(problem, batch: int, seed: int = 3) -> np.ndarray:
     return problem.uω0[None, :, 0] * np.exp(1j * phases)  # (batch, nt)
 
 
-def time_path(problem, steps, precision, batch, k, warmup):
+def time_path(problem, steps, precision, batch, k, warmup, u_host=None):
     """K event-timed launches (hipEvents on the launch stream, inside libfiberhip)."""
     import fiberhip
 
     plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch)
-    u0 = plan.to_device(launches(problem, batch)[:, :, None])
+    u0 = plan.to_device(launches(problem, batch)[:, :, None] if u_host is None else u_host)
     out = fiberhip.DeviceArray(plan.context, u0.nbytes)
     for _ in range(warmup):
         plan.forward_device(batch, u0.ptr, out.ptr)
@@ -93,6 +97,16 @@ def time_path(problem, steps, precision, batch, k, warmup):
     energies = plan.energy_device(batch, out.ptr)
     return dict(plan=plan, wall_s=wall, kernel_ms=float(np.mean(ms)), kernel_ms_p50=float(np.median(ms)),
                 energies=energies)
+
+
+def config1_launches(batch: int) -> np.ndarray:
+    """Config-1 requests: seeded Dirichlet(1) mode weights and uniform phases per request."""
+    from fiberhip import grin
+
+    out = []
+    for b in range(batch):
+        out.append(grin.grin_problem(6, seed=1000 + b).uω0)
+    return np.stack(out)
 
 
 def time_gradient(problem, steps, precision, batch, k, warmup, mask):
@@ -198,7 +212,8 @@ def main() -> None:
     ap.add_argument("--batch", type=int, default=256, help="gradients per launch (config 2)")
     ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (config 0)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "gradient", "forward"], default="all")
+    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode"], default="all")
+    ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (config 1)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -294,6 +309,34 @@ def main() -> None:
                     "config": {"workload": fwd["workload"]}, "forward": fwd}
         else:
             line["forward_config0"] = fwd
+    if args.only in ("all", "multimode"):
+        problem, steps = workload("config1")
+        nt, m = problem.nt, problem.modes
+        uh = config1_launches(args.mm_batch)
+        res = {p: time_path(problem, steps, p, args.mm_batch, 1, 0, u_host=uh) for p in ("fp32", "fp64")}
+        e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
+        per_mode = np.max(np.abs(e32 - e64) / e64, axis=0)
+        flops = forward_flops(nt, m, steps)
+        mm = {
+            "workload": f"config1: GRIN 50 um / NA 0.2, 0.3 m, M=6 HG modes (256x256), Nt={nt}, 12 ps, 10 nJ sech2 "
+                        f"100 fs, seeded Dirichlet launches, {steps} RK4IP steps",
+            "batch_solves_per_step": args.mm_batch,
+            "fp32_solves_per_s": args.mm_batch / (res["fp32"]["kernel_ms"] / 1e3),
+            "fp64_solves_per_s": args.mm_batch / (res["fp64"]["kernel_ms"] / 1e3),
+            "fp32_ms_per_launch": res["fp32"]["kernel_ms"],
+            "fp64_ms_per_launch": res["fp64"]["kernel_ms"],
+            "fp32_vs_fp64_max_rel_energy_err_per_mode": [float(x) for x in per_mode],
+            "fp32_roofline_frac": args.mm_batch * flops / (res["fp32"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp32"],
+            "fp64_roofline_frac": args.mm_batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp64"],
+            "flops_per_solve": flops,
+        }
+        if args.only == "multimode":
+            line = {"metric": "fwd solves/sec (FP32-bulk batched path, config 1)", "value": mm["fp32_solves_per_s"],
+                    "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": mm["fp32_ms_per_launch"],
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                    "data": "synthetic", "config": {"workload": mm["workload"]}, "multimode": mm}
+        else:
+            line["forward_config1"] = mm
     print(json.dumps(line), flush=True)
 
 

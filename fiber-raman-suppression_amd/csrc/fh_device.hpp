@@ -178,14 +178,14 @@ struct Buf {
         : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000))
     {
     }
-    template <typename V> __device__ __forceinline__ V ld(uint32_t voff, uint32_t soff) const
+    template <typename V, int AUX = 0> __device__ __forceinline__ V ld(uint32_t voff, uint32_t soff) const
     {
         if constexpr (sizeof(V) == 16)
-            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, AUX));
         else if constexpr (sizeof(V) == 8)
-            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
         else
-            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
     }
     template <typename V> __device__ __forceinline__ void st(V v, uint32_t voff, uint32_t soff) const
     {
@@ -198,6 +198,24 @@ struct Buf {
         } else {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
         }
+    }
+};
+
+// A field inside a larger buffer: the per-lane byte offset is shifted by
+// `off`.  Kernels keep one descriptor per buffer region for their whole
+// lifetime and address sub-arrays this way: rebuilding descriptors per field
+// with a runtime index lets the compiler recycle the descriptor SGPRs right
+// after a load is issued, which made solves lose determinism.
+struct Fld {
+    const Buf& b;
+    uint32_t off;
+    template <typename W> __device__ __forceinline__ W ld(uint32_t v, uint32_t so) const
+    {
+        return b.template ld<W>(v + off, so);
+    }
+    template <typename W> __device__ __forceinline__ void st(W x, uint32_t v, uint32_t so) const
+    {
+        b.st(x, v + off, so);
     }
 };
 

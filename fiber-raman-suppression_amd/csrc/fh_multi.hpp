@@ -67,8 +67,12 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
     const size_t sol = blockIdx.x;
     const int Q = a.q, QC = (a.q + 1) / 2;
+    // One descriptor for the whole workgroup slice, fixed for the kernel's
+    // lifetime; field f is addressed through the per-lane offset
+    // vo + f * N * sizeof(V) (see Fld).
     V* slice = a.work + sol * (size_t)TR::FIELDS * N;
-    auto fld = [&](int f) { return Buf(slice + (size_t)f * N, N * sizeof(V)); };
+    const Buf sl(slice, (uint32_t)((size_t)TR::FIELDS * N * sizeof(V)));
+    auto fld = [&](int f) { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; };
     const Buf disp(a.disp, M * N * 8), hn(a.hn, N * sizeof(V)), shn(a.shn, N * sizeof(R));
     const Buf tw(a.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
     const Buf in(a.u0 + sol * M * N, M * N * sizeof(V));
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
 
     // y = st = u0
     for (int m = 0; m < M; ++m) {
-        const Buf y = fld(m), st = fld(M + m);
+        const Fld y = fld(m), st = fld(M + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
@@ -96,12 +100,12 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
             // 1. u_m = fft(cis(D_m zs) y_m)
 #pragma unroll 1
             for (int m = 0; m < M; ++m) {
-                const Buf y = fld(m), u = fld(3 * M + m);
+                const Fld y = fld(m), u = fld(3 * M + m);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     R c, sn;
                     cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, c, sn);
-                    const V x = y.ld<V>(vo, CV::soff(k));
+                    const V x = y.template ld<V>(vo, CV::soff(k));
                     v[k] = V{x.x * c - x.y * sn, x.x * sn + x.y * c};
                 }
                 block_fft<R, N, T, -1, TR::PLANAR>(v, tw, lds);
@@ -139,16 +143,16 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
             // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))
 #pragma unroll 1
             for (int c = 0; c < QC; ++c) {
-                const Buf hq = fld(4 * M + c);
+                const Fld hq = fld(4 * M + c);
 #pragma unroll
-                for (int k = 0; k < P; ++k) v[k] = hq.ld<V>(vo, CV::soff(k));
+                for (int k = 0; k < P; ++k) v[k] = hq.template ld<V>(vo, CV::soff(k));
                 block_fft<R, N, T, -1, TR::PLANAR>(v, tw, lds);
 #pragma unroll
                 for (int k = 0; k < P; ++k) v[k] = cmul(v[k], hn.ld<V>(vo, CV::soff(k)));
                 block_fft<R, N, T, +1, TR::PLANAR>(v, tw, lds);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    const V d = hq.ld<V>(vo, CV::soff(k));
+                    const V d = hq.template ld<V>(vo, CV::soff(k));
                     hq.st(V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y}, vo, CV::soff(k));
                 }
             }
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
                     v[k] = acc;
                 }
                 block_fft<R, N, T, +1, TR::PLANAR>(v, tw, lds);
-                const Buf y = fld(i), st = fld(M + i), ac = fld(2 * M + i);
+                const Fld y = fld(i), st = fld(M + i), ac = fld(2 * M + i);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     R c, sn;
@@ -177,8 +181,8 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
                     const R sc = shn.ld<R>(CR::voff(), CR::soff(k));
                     const V e = V{v[k].x * sc, v[k].y * sc};
                     const V kk = V{sn * e.x - c * e.y, sn * e.y + c * e.x};
-                    const V x = st.ld<V>(vo, CV::soff(k));
-                    V cc = s != 0 ? ac.ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
+                    const V x = st.template ld<V>(vo, CV::soff(k));
+                    V cc = s != 0 ? ac.template ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
                     V yn;
                     if (s == 0) {
                         cc = kk;
@@ -203,12 +207,12 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     int bad = 0;
     const Buf out(a.out + sol * M * N, M * N * sizeof(V));
     for (int m = 0; m < M; ++m) {
-        const Buf st = fld(M + m);
+        const Fld st = fld(M + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R c, sn;
             cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), a.length, c, sn);
-            const V x = st.ld<V>(vo, CV::soff(k));
+            const V x = st.template ld<V>(vo, CV::soff(k));
             const V o = V{x.x * c - x.y * sn, x.x * sn + x.y * c};
             bad |= !(isfinite(o.x) && isfinite(o.y));
             out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));

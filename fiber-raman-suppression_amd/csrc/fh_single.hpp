@@ -119,7 +119,7 @@ template <typename R, int N, int T> struct SmCtx {
 // the stage's time field u and weight h are stored to tu / th.
 template <typename R, int N, int T, bool TAPE>
 __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, const SmTables<R, N>& tb,
-                                       typename SmCtx<R, N, T>::FU& ut, const Buf& tu, const Buf& th, void* lds)
+                                       typename SmCtx<R, N, T>::FU& ut, const Fld& tu, const Fld& th, void* lds)
 {
     using V = C2<R>;
     using TR = SmTraits<R, N, T>;
@@ -176,10 +176,13 @@ __device__ __forceinline__ void sm_rhs(C2<R>* v, double zs, const SmArgs<R>& a, 
 }
 
 // One RK4 step from state st (v holds st on entry and the new state on exit).
+// tape_u / tape_h: the solve's tape regions (fixed descriptors); the stage's
+// slot is addressed through the per-lane offset (slot0 + s) * N * element.
 template <typename R, int N, int T, bool TAPE>
 __device__ __forceinline__ void sm_step(C2<R>* v, int step, const SmArgs<R>& a, const SmTables<R, N>& tb,
                                         typename SmCtx<R, N, T>::FS& st, typename SmCtx<R, N, T>::FS& ac,
-                                        typename SmCtx<R, N, T>::FU& ut, C2<R>* tape_u, R* tape_h, void* lds)
+                                        typename SmCtx<R, N, T>::FU& ut, const Buf& tape_u, const Buf& tape_h,
+                                        int slot0, void* lds)
 {
     using V = C2<R>;
     constexpr int P = N / T;
@@ -189,8 +192,8 @@ __device__ __forceinline__ void sm_step(C2<R>* v, int step, const SmArgs<R>& a, 
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
         const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
-        const Buf tu(TAPE ? tape_u + (size_t)s * N : tape_u, N * sizeof(V));
-        const Buf th(TAPE ? tape_h + (size_t)s * N : tape_h, N * sizeof(R));
+        const Fld tu{tape_u, (uint32_t)(slot0 + s) * (uint32_t)(N * sizeof(V))};
+        const Fld th{tape_h, (uint32_t)(slot0 + s) * (uint32_t)(N * sizeof(R))};
         sm_rhs<R, N, T, TAPE>(v, zs, a, tb, ut, tu, th, lds);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
@@ -239,7 +242,8 @@ __global__ __launch_bounds__(T) void k_rk4ip_single(SmArgs<R> a)
         v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
         st.set(k, v[k]);
     }
-    for (int step = 0; step < a.steps; ++step) sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, nullptr, nullptr, lds);
+    const Buf none(a.u0, 0);
+    for (int step = 0; step < a.steps; ++step) sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, none, none, 0, lds);
     int bad = 0;
     const Buf out(a.out + sol * N, N * sizeof(V));
     using CD = Cyc<double, T>;
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_single(SmArgs<R> a)
 // gy of the stage output, on exit the cotangent gx of the stage input.
 template <typename R, int N, int T>
 __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, const SmTables<R, N>& tb,
-                                       typename SmCtx<R, N, T>::FU& gu, const Buf& tu, const Buf& th, void* lds)
+                                       typename SmCtx<R, N, T>::FU& gu, const Fld& tu, const Fld& th, void* lds)
 {
     using V = C2<R>;
     using TR = SmTraits<R, N, T>;
@@ -285,8 +289,8 @@ __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, 
     block_fft<R, N, T, -1, TR::PLANAR>(v, tb.tw, lds);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const V u = tu.ld<V>(CV::voff(), CV::soff(k));
-        const R hh = th.ld<R>(CR::voff(), CR::soff(k));
+        const V u = tu.template ld<V>(CV::voff(), CV::soff(k));
+        const R hh = th.template ld<R>(CR::voff(), CR::soff(k));
         const V G = v[k];
         gu.set(k, V{hh * G.x, hh * G.y});
         av[k] = G.x * u.x + G.y * u.y;
@@ -298,7 +302,7 @@ __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, 
     block_fft<R, N, T, +1, TR::PLANAR>(v, tb.tw, lds);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const V u = tu.ld<V>(CV::voff(), CV::soff(k));
+        const V u = tu.template ld<V>(CV::voff(), CV::soff(k));
         const R b = v[k].x + a.one_m_fr * av[k];
         const R f = 2 * a.gamma * b;
         const V g = gu.get(k);
@@ -361,9 +365,12 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
     typename CX::FS lam(Buf(gwork + 2 * N, N * sizeof(V))), lacc(Buf(gwork + 3 * N, N * sizeof(V)));
     typename CX::FU ut(Buf(gwork + (size_t)US * N, N * sizeof(V)));
     typename CX::FU gu(Buf(gwork + (size_t)(US + 1) * N, N * sizeof(V)));
-    V* ckpt = a.ckpt + sol * (size_t)a.nseg * N;
-    V* tape_u = a.tape_u + sol * (size_t)4 * a.every * N;
-    R* tape_h = a.tape_h + sol * (size_t)4 * a.every * N;
+    // Fixed descriptors per solve region; checkpoints and tape slots are
+    // addressed through the per-lane offset.
+    const Buf ckpt(a.ckpt + sol * (size_t)a.nseg * N, (uint32_t)((size_t)a.nseg * N * sizeof(V)));
+    const Buf tape_u(a.tape_u + sol * (size_t)4 * a.every * N, (uint32_t)((size_t)4 * a.every * N * sizeof(V)));
+    const Buf tape_h(a.tape_h + sol * (size_t)4 * a.every * N, (uint32_t)((size_t)4 * a.every * N * sizeof(R)));
+    const Buf none(a.u0, 0);
     const double h = a.h;
 
     // forward with checkpoints
@@ -375,11 +382,11 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
     }
     for (int step = 0; step < a.steps; ++step) {
         if (step % a.every == 0) {
-            const Buf ck(ckpt + (size_t)(step / a.every) * N, N * sizeof(V));
+            const Fld ck{ckpt, (uint32_t)(step / a.every) * (uint32_t)(N * sizeof(V))};
 #pragma unroll
             for (int k = 0; k < P; ++k) ck.st(v[k], CV::voff(), CV::soff(k));
         }
-        sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, nullptr, nullptr, lds);
+        sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, none, none, 0, lds);
     }
     // output and terminal seed
     const Buf out(a.out + sol * N, N * sizeof(V));
@@ -426,19 +433,16 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
     for (int sg = a.nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * a.every;
         const int s1 = min(s0 + a.every, a.steps);
-        const Buf ck(ckpt + (size_t)sg * N, N * sizeof(V));
+        const Fld ck{ckpt, (uint32_t)sg * (uint32_t)(N * sizeof(V))};
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            v[k] = ck.ld<V>(CV::voff(), CV::soff(k));
+            v[k] = ck.template ld<V>(CV::voff(), CV::soff(k));
             st.set(k, v[k]);
         }
         for (int step = s0; step < s1; ++step)
-            sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u + (size_t)4 * (step - s0) * N,
-                                   tape_h + (size_t)4 * (step - s0) * N, lds);
+            sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, 4 * (step - s0), lds);
         for (int step = s1 - 1; step >= s0; --step) {
             const double z = step * h;
-            V* tus = tape_u + (size_t)4 * (step - s0) * N;
-            R* ths = tape_h + (size_t)4 * (step - s0) * N;
 #pragma unroll 1
             for (int s = 3; s >= 0; --s) {
                 const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
@@ -450,8 +454,9 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
                     if (s == 3) lacc.set(k, l);
                     v[k] = V{cl * l.x + cg * v[k].x, cl * l.y + cg * v[k].y};
                 }
-                const Buf tu(tus + (size_t)s * N, N * sizeof(V));
-                const Buf th(ths + (size_t)s * N, N * sizeof(R));
+                const int slot = 4 * (step - s0) + s;
+                const Fld tu{tape_u, (uint32_t)slot * (uint32_t)(N * sizeof(V))};
+                const Fld th{tape_h, (uint32_t)slot * (uint32_t)(N * sizeof(R))};
                 sm_vjp<R, N, T>(v, zs, a, tb, gu, tu, th, lds);
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
