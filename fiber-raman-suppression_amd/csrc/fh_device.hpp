@@ -168,6 +168,24 @@ template <int N, int T> struct FftPlan {
 __host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 4); }
 template <int N> struct LdsSize { static constexpr int value = N + N / 16; };
 
+// Ordering point: wait until every memory operation this wave has issued
+// (vector, scratch, LDS, scalar) has completed.  Kernels place one at each
+// LDS exchange and wherever a lane re-reads global scratch it has just
+// stored, before any transform separates the two.  Without these points the
+// FP64 adjoint kernels returned results that differed between identical
+// solves of one launch (relative 1e-7 in lam0, varying from run to run),
+// although a dataflow check of the compiler's s_waitcnt placement finds no
+// missing wait; compiling with every wait forced to zero also removes the
+// difference.  Measured cost: within run-to-run noise.
+__device__ __forceinline__ void drain_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Largest complex (interleaved) exchange buffer a kernel uses before it
+// switches to the planar exchange.  FP64 kernels stay at or below 128 KiB:
+// the 1024-thread FP64 kernels with a 139264-byte interleaved buffer
+// (8192 points) gave run-to-run differences in the 1e-8..1e-6 range between
+// identical solves, which the planar layout (69632 bytes) does not show.
+template <typename R> struct LdsBudget { static constexpr size_t bytes = sizeof(R) == 8 ? 128 * 1024 : 160 * 1024; };
+
 // Raw buffer access with a per-lane byte offset (VGPR) and a uniform byte
 // offset (SGPR / immediate).  Keeping the k*T part of a cyclic index in the
 // uniform offset means a whole cyclic set needs one address VGPR instead of
@@ -241,6 +259,7 @@ template <typename R, int N, int T, int RAD, int NS, bool PLANAR> struct Exchang
     static __device__ __forceinline__ void run(V* v, const int* base, void* lds_raw)
     {
         const int tid = threadIdx.x;
+        drain_memory();
         if constexpr (!PLANAR) {
             V* lds = reinterpret_cast<V*>(lds_raw);
 #pragma unroll

@@ -46,7 +46,7 @@ template <typename R> struct MmArgs {
 template <typename R, int N, int T, int M> struct MmTraits {
     static constexpr int P = N / T;
     static constexpr int NP = M * (M + 1) / 2;  // input pairs k <= l
-    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 2 * sizeof(R) > 160 * 1024;
+    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 2 * sizeof(R) > LdsBudget<R>::bytes;
     static constexpr size_t LDS_BYTES = (size_t)LdsSize<N>::value * (PLANAR ? sizeof(R) : 2 * sizeof(R));
     static constexpr int QMAX = M * M;
     static constexpr int QCMAX = (QMAX + 1) / 2;
@@ -97,6 +97,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
             const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
+            drain_memory();  // re-reads y / st / ac of the previous stage
             // 1. u_m = fft(cis(D_m zs) y_m)
 #pragma unroll 1
             for (int m = 0; m < M; ++m) {
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
 #pragma unroll
                 for (int k = 0; k < P; ++k) u.st(v[k], vo, CV::soff(k));
             }
+            drain_memory();  // re-reads u of step 1
             // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p
 #pragma unroll 1
             for (int k = 0; k < P; ++k) {
@@ -140,6 +142,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
                     fld(4 * M + c).st(V{d0, d1}, vo, CV::soff(k));
                 }
             }
+            drain_memory();  // re-reads hq of step 2
             // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))
 #pragma unroll 1
             for (int c = 0; c < QC; ++c) {
@@ -156,6 +159,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
                     hq.st(V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y}, vo, CV::soff(k));
                 }
             }
+            drain_memory();  // re-reads hq of step 3
             // 4. per mode: eta_i = sum_j h_ij u_j; k_i; RK4 bookkeeping
 #pragma unroll 1
             for (int i = 0; i < M; ++i) {

@@ -63,13 +63,13 @@ template <int N, int T> struct SmTraits<double, N, T> {
     static constexpr bool STATE_REGS = false;
     static constexpr bool UT_REGS = T <= 512 && N / T <= 8;
     static constexpr bool KEEP_PHASE = false;
-    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 16 > 160 * 1024;
+    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 16 > LdsBudget<double>::bytes;
 };
 template <int N, int T> struct SmTraits<float, N, T> {
     static constexpr bool STATE_REGS = T <= 512;
     static constexpr bool UT_REGS = true;
     static constexpr bool KEEP_PHASE = true;
-    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 8 > 160 * 1024;
+    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 8 > LdsBudget<float>::bytes;
 };
 
 template <typename R, int N, int T> struct SmLayout {
@@ -261,6 +261,8 @@ __global__ __launch_bounds__(T) void k_rk4ip_single(SmArgs<R> a)
 
 // Pullback of one right-hand-side evaluation.  On entry v holds the cotangent
 // gy of the stage output, on exit the cotangent gx of the stage input.
+// Each tape slot is pulled back exactly once, so the slot's h entry is
+// overwritten with the real field a = Re(conj(G) u) after h is used.
 template <typename R, int N, int T>
 __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, const SmTables<R, N>& tb,
                                        typename SmCtx<R, N, T>::FU& gu, const Fld& tu, const Fld& th, void* lds)
@@ -272,7 +274,6 @@ __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, 
     using CR = Cyc<R, T>;
     constexpr int P = N / T;
     R cs[TR::KEEP_PHASE ? P : 1], sn[TR::KEEP_PHASE ? P : 1];
-    R av[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         R c, s;
@@ -293,8 +294,9 @@ __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, 
         const R hh = th.template ld<R>(CR::voff(), CR::soff(k));
         const V G = v[k];
         gu.set(k, V{hh * G.x, hh * G.y});
-        av[k] = G.x * u.x + G.y * u.y;
-        v[k] = V{av[k], (R)0};
+        const R ak = G.x * u.x + G.y * u.y;
+        th.st(ak, CR::voff(), CR::soff(k));  // h of this slot is consumed; the slot now holds a
+        v[k] = V{ak, (R)0};
     }
     block_fft<R, N, T, -1, TR::PLANAR>(v, tb.tw, lds);
 #pragma unroll
@@ -303,7 +305,7 @@ __device__ __forceinline__ void sm_vjp(C2<R>* v, double zs, const SmArgs<R>& a, 
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const V u = tu.template ld<V>(CV::voff(), CV::soff(k));
-        const R b = v[k].x + a.one_m_fr * av[k];
+        const R b = v[k].x + a.one_m_fr * th.template ld<R>(CR::voff(), CR::soff(k));
         const R f = 2 * a.gamma * b;
         const V g = gu.get(k);
         v[k] = V{g.x + f * u.x, g.y + f * u.y};
@@ -445,6 +447,7 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
             const double z = step * h;
 #pragma unroll 1
             for (int s = 3; s >= 0; --s) {
+                drain_memory();  // lam / lacc were stored by the previous stage
                 const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
                 const R cl = (s == 0 || s == 3) ? c6 : c3;   // weight of k_s in the update
                 const R cg = s == 3 ? (R)0 : (s == 2 ? c1 : c2);  // dy_{s+1}/dk_s

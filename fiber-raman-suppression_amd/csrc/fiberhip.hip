@@ -228,8 +228,8 @@ int check_desc(const fh_problem_desc* d)
     if (d->modes == 1) {
         if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
     } else if (!fh::multi_supported(d->nt, d->modes, d->precision == FH_PREC_FP64)) {
-        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {2,3,4,6,10} and nt in [256, 8192] (FP64) or "
-                                     "[256, 16384] (FP32)");
+        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {3,4,6,10} with nt in [256, 8192] (FP64) and "
+                                     "M in {2,3,4,6,10} with nt in [256, 16384] (FP32)");
     }
     return FH_OK;
 }

@@ -76,9 +76,15 @@ def test_config2_reduced_fp64_and_fd(gpu_context):
     steps = 200
     phase = _phases(problem, 1, 2)[0]
     launch = problem.uω0 * np.exp(1j * phase)[:, None]
-    plan = fiberhip.Plan.from_problem(problem, steps=steps, precision="fp64", band_mask=mask, checkpoint_every=50)
+    plan = fiberhip.Plan.from_problem(problem, steps=steps, precision="fp64", band_mask=mask, checkpoint_every=50,
+                                      max_batch=8)
     print("case: config2 reduced fp64 nt=16384 steps=200", flush=True)
     res = plan.gradient(launch)
+    # eight identical solves in one launch return identical bits
+    many = plan.gradient(np.stack([launch] * 8))
+    for b in range(8):
+        assert np.array_equal(many["lam0"][b], res["lam0"]), b
+        assert np.array_equal(many["grad"][b], res["grad"]), b
     fib = oracle_fiber(problem)
     uL = oracle.rk4_forward(fib, 2.0, steps, launch)
     J, seed = R.band_cost(uL, mask)
