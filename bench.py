@@ -6,7 +6,7 @@ with seeded Taylor spectral phases, as in the candidate sweep).  Inputs are
 resident in HBM before the timed region; the timed region is K launches
 bracketed by synchronisations, timed with HIP events on the launch stream.
 
-Prints one JSON line (see README / DESIGN for the fields).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -52,16 +52,16 @@ def gradient_flops(nt: int, m: int, steps: int) -> float:
 def workload(name: str):
     import fiberhip
 
-    if name == "config0":
+    if name == "smf28-fwd-8192":
         fib = fiberhip.Fiber(length_m=0.5, power_w=0.1, preset="SMF28", beta_order=3)
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=8192, time_window_ps=10.0),
                                          raman_threshold_thz=None)
         return problem, 200
-    if name == "config1":
+    if name == "grin6-fwd-4096":
         from fiberhip import grin
 
         return grin.grin_problem(6), 4000
-    if name == "config2":
+    if name == "smf28-grad-16384":
         fib = fiberhip.Fiber(length_m=2.0, power_w=0.2, preset="SMF28", beta_order=3)
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=16384, time_window_ps=47.061, policy="exact"),
                                          raman_threshold_thz=None)
@@ -99,8 +99,8 @@ def time_path(problem, steps, precision, batch, k, warmup, u_host=None):
                 energies=energies)
 
 
-def config1_launches(batch: int) -> np.ndarray:
-    """Config-1 requests: seeded Dirichlet(1) mode weights and uniform phases per request."""
+def grin_launches(batch: int) -> np.ndarray:
+    """GRIN M=6 requests: seeded Dirichlet(1) mode weights and uniform phases per request."""
     from fiberhip import grin
 
     out = []
@@ -162,7 +162,7 @@ def cpu_baseline_gradient(problem, steps, mask, slice_steps=200):
     wall = time.perf_counter() - t
     value = threads / wall * slice_steps / steps
     return dict(value=value, unit="grads/s", cores=threads, kind="port",
-                sample=f"{threads} config-2 gradients (forward + discrete adjoint, checkpoint every 50) over the first "
+                sample=f"{threads} SMF-28 2 m gradients (forward + discrete adjoint, checkpoint every 50) over the first "
                        f"{slice_steps} of {steps} RK4 steps at Nt={problem.nt}, one per thread, {wall:.1f} s wall; "
                        f"rate scaled by {slice_steps}/{steps}")
 
@@ -199,7 +199,7 @@ def cpu_baseline(problem, steps, budget_s=20.0):
         th.join()
     wall = time.perf_counter() - t
     return dict(value=done[0] / wall, unit="solves/s", cores=threads, kind="port",
-                sample=f"{done[0]} config-0 solves ({steps} RK4 steps, Nt={problem.nt}) on {threads} threads, "
+                sample=f"{done[0]} SMF-28 0.5 m solves ({steps} RK4 steps, Nt={problem.nt}) on {threads} threads, "
                        f"one solve per thread at a time; single-thread latency {single:.3f} s",
                 single_thread_s=single)
 
@@ -209,11 +209,11 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed launches per path")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256, help="gradients per launch (config 2)")
-    ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (config 0)")
+    ap.add_argument("--batch", type=int, default=256, help="gradients per launch (SMF-28 2 m gradient workload)")
+    ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (SMF-28 0.5 m forward workload)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode"], default="all")
-    ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (config 1)")
+    ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -231,13 +231,13 @@ def main() -> None:
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (independent solves shard across GPUs without "
-              "communication; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "communication); run with --gpus 1", file=sys.stderr)
         raise SystemExit(2)
     import fiberhip
 
     line = {}
     if args.only in ("all", "gradient"):
-        problem, steps = workload("config2")
+        problem, steps = workload("smf28-grad-16384")
         nt, m = problem.nt, problem.modes
         mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
         g = {p: time_gradient(problem, steps, p, args.batch, args.steps, args.warmup, mask) for p in ("fp32", "fp64")}
@@ -261,7 +261,7 @@ def main() -> None:
             "vs_baseline": None,
             "dtype": "f32",
             "data": "synthetic",
-            "config": {"workload": f"config2: SMF-28 2 m, 0.2 W, 185 fs sech2, Nt={nt}, T={problem.sim['time_window']} ps, "
+            "config": {"workload": f"SMF-28 2 m, 0.2 W, 185 fs sech2, Nt={nt}, T={problem.sim['time_window']} ps, "
                                    f"{steps} RK4IP steps, checkpoint every 50, band -16..-10 THz objective, "
                                    f"gradient w.r.t. the full-grid spectral phase",
                        "batch_gradients_per_step": args.batch, "modes": m, "nt": nt},
@@ -282,13 +282,13 @@ def main() -> None:
             cb = cpu_baseline_gradient(problem, steps, mask)
             line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     if args.only in ("all", "forward"):
-        problem, steps = workload("config0")
+        problem, steps = workload("smf28-fwd-8192")
         nt, m = problem.nt, problem.modes
         res = {p: time_path(problem, steps, p, args.fwd_batch, args.steps, args.warmup) for p in ("fp32", "fp64")}
         e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
         flops = forward_flops(nt, m, steps)
         fwd = {
-            "workload": f"config0: SMF-28 0.5 m, 0.1 W, Nt={nt}, T={problem.sim['time_window']} ps, {steps} RK4IP steps",
+            "workload": f"SMF-28 0.5 m, 0.1 W, Nt={nt}, T={problem.sim['time_window']} ps, {steps} RK4IP steps",
             "batch_solves_per_step": args.fwd_batch,
             "fp32_solves_per_s": args.fwd_batch / (res["fp32"]["kernel_ms"] / 1e3),
             "fp64_solves_per_s": args.fwd_batch / (res["fp64"]["kernel_ms"] / 1e3),
@@ -308,17 +308,17 @@ def main() -> None:
                     "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                     "config": {"workload": fwd["workload"]}, "forward": fwd}
         else:
-            line["forward_config0"] = fwd
+            line["forward_smf28_8192"] = fwd
     if args.only in ("all", "multimode"):
-        problem, steps = workload("config1")
+        problem, steps = workload("grin6-fwd-4096")
         nt, m = problem.nt, problem.modes
-        uh = config1_launches(args.mm_batch)
+        uh = grin_launches(args.mm_batch)
         res = {p: time_path(problem, steps, p, args.mm_batch, 1, 0, u_host=uh) for p in ("fp32", "fp64")}
         e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
         per_mode = np.max(np.abs(e32 - e64) / e64, axis=0)
         flops = forward_flops(nt, m, steps)
         mm = {
-            "workload": f"config1: GRIN 50 um / NA 0.2, 0.3 m, M=6 HG modes (256x256), Nt={nt}, 12 ps, 10 nJ sech2 "
+            "workload": f"GRIN 50 um / NA 0.2, 0.3 m, M=6 HG modes (256x256), Nt={nt}, 12 ps, 10 nJ sech2 "
                         f"100 fs, seeded Dirichlet launches, {steps} RK4IP steps",
             "batch_solves_per_step": args.mm_batch,
             "fp32_solves_per_s": args.mm_batch / (res["fp32"]["kernel_ms"] / 1e3),
@@ -331,12 +331,12 @@ def main() -> None:
             "flops_per_solve": flops,
         }
         if args.only == "multimode":
-            line = {"metric": "fwd solves/sec (FP32-bulk batched path, config 1)", "value": mm["fp32_solves_per_s"],
+            line = {"metric": "fwd solves/sec (FP32-bulk batched path, GRIN M=6)", "value": mm["fp32_solves_per_s"],
                     "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": mm["fp32_ms_per_launch"],
                     "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                     "data": "synthetic", "config": {"workload": mm["workload"]}, "multimode": mm}
         else:
-            line["forward_config1"] = mm
+            line["forward_grin6_4096"] = mm
     print(json.dumps(line), flush=True)
 
 

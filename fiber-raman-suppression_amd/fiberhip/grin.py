@@ -110,7 +110,7 @@ def grin_problem(modes: int = 6, nt: int = 4096, time_window_ps: float = 12.0, l
                  energy_j: float = 10e-9, fwhm_s: float = 100e-15, seed: int = 1, nx: int = 256,
                  window_um: float = 80.0, radius_um=25.0, core_na=0.2, wavelength_m=1550e-9, fR=0.18,
                  launch=None) -> FiberFieldProblem:
-    """Config-1 problem: parabolic GRIN (50 um core, NA 0.2), M HG modes on an
+    """Default problem: parabolic GRIN (50 um core, NA 0.2), M HG modes on an
     nx^2 grid, sech^2 pulse of energy_j split over modes by Dirichlet(1)
     weights with uniform phases (seeded), initial_modes normalised
     (physics_models.jl:289-297)."""

@@ -11,10 +11,10 @@
  *   get_disp_sim_params / get_disp_fiber_params_user_defined /
  *   _single_oscillator_raman_fields          src/helpers/helpers.jl:56-84,245-282,97-130
  *   get_p_disp_mmf (selfsteep)               src/simulation/simulate_disp_mmf.jl:87-114
- * A Julia caller binds the entry points below with `ccall` (see
- * INTEGRATION.md); arrays are Julia-native column-major, so they pass without
- * copies: a field of nt samples and m modes is m*nt complex doubles stored as
- * interleaved (re, im) pairs, sample t of mode k at element k*nt + t, and a
+ * A Julia caller binds the entry points below with `ccall`; arrays are
+ * Julia-native column-major, so they pass without copies: a field of nt
+ * samples and m modes is m*nt complex doubles stored as interleaved (re, im)
+ * pairs, sample t of mode k at element k*nt + t, and a
  * batch of fields is batch such blocks back to back.
  *
  * All functions return FH_OK (0) or one of the FH_E* codes; the message of

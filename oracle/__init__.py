@@ -8,7 +8,7 @@ CPU baseline), never the thing measured or shipped.  The product package
 Two independent restatements live here:
 
 * ``fiberoracle.c`` (loaded below through ctypes): forward RHS, fixed-step
-  global-frame RK4 (the config-0 oracle), its discrete adjoint and the
+  global-frame RK4 (scalar_crosscheck.jl:8-68), its discrete adjoint and the
   reference's continuous adjoint RHS, with a self-written radix-2 FFT.
 * ``np_ref.py``: numpy restatement of the setup helpers and the same
   RHS / RK4 / VJP, using numpy's FFT.

@@ -14,7 +14,7 @@
  * mode k at [k*nt + t].  gamma[i,j,k,l] (1-based in Julia) is stored at
  * i + m*(j + m*(k + m*l)) with 0-based indices.
  *
- * Conventions (Appendix A of the survey, restated from the source):
+ * Conventions (restated from the source):
  *   fft  = unnormalised, kernel exp(-2 pi i k n / N)      (FFTW.fft)
  *   ifft = exp(+2 pi i k n / N) scaled by 1/N              (FFTW.ifft)
  */

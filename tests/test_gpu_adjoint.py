@@ -63,8 +63,8 @@ def test_fp64_gradient_matches_oracle(gpu_context, every):
     assert np.isnan(seeded["cost"])
 
 
-def test_config2_reduced_fp64_and_fd(gpu_context):
-    """Config 2 geometry (SMF-28, 2 m, 0.2 W, Nt=16384, 47.061 ps, band
+def test_smf28_16384_reduced_fp64_and_fd(gpu_context):
+    """SMF-28 gradient geometry (SMF-28, 2 m, 0.2 W, Nt=16384, 47.061 ps, band
     -16..-10 THz, checkpoint every 50) with 200 steps for the oracle run;
     central differences on random bins with h = 1e-4 rad."""
     import fiberhip
@@ -78,7 +78,7 @@ def test_config2_reduced_fp64_and_fd(gpu_context):
     launch = problem.uω0 * np.exp(1j * phase)[:, None]
     plan = fiberhip.Plan.from_problem(problem, steps=steps, precision="fp64", band_mask=mask, checkpoint_every=50,
                                       max_batch=8)
-    print("case: config2 reduced fp64 nt=16384 steps=200", flush=True)
+    print("case: smf28-16384 reduced fp64 nt=16384 steps=200", flush=True)
     res = plan.gradient(launch)
     # eight identical solves in one launch return identical bits
     many = plan.gradient(np.stack([launch] * 8))

@@ -22,34 +22,34 @@ def _oracle_out(problem, steps, launch=None):
     return oracle.rk4_forward(oracle_fiber(problem), problem.fiber["L"], steps, u0)
 
 
-def test_config0_fp64_matches_oracle(gpu_context):
-    """Config 0: SMF-28, 0.5 m, 0.1 W, 185 fs sech^2, Nt=8192 on the resolved
+def test_smf28_8192_fp64_matches_oracle(gpu_context):
+    """SMF-28, 0.5 m, 0.1 W, 185 fs sech^2, Nt=8192 on the resolved
     23.531 ps window, 200 fixed RK4 steps; FP64 within 1e-10."""
     import fiberhip
 
     problem = smf_problem(nt=8192, window_ps=10.0, policy="auto_if_undersized")
     assert problem.nt == 8192 and abs(problem.sim["time_window"] - 23.531) < 1e-12
-    print("case: config0 fp64 nt=8192 steps=200", flush=True)
+    print("case: smf28-8192 fp64 nt=8192 steps=200", flush=True)
     gpu = fiberhip.scalar_rk4_output(problem, 200, precision="fp64")
     ref = _oracle_out(problem, 200)
     e_rel = mode_energy_rel(gpu, ref)
     f_rel = rel(gpu, ref)
-    print(f"config0 fp64: energy rel {e_rel}, field rel {f_rel:.3e}", flush=True)
+    print(f"smf28-8192 fp64: energy rel {e_rel}, field rel {f_rel:.3e}", flush=True)
     assert np.all(e_rel <= 1e-10)
     assert f_rel <= 1e-10
 
 
-def test_config0_fp32_bulk_error(gpu_context):
-    """FP32-bulk path on config 0 within 1e-5 relative spectral energy."""
+def test_smf28_8192_fp32_bulk_error(gpu_context):
+    """FP32-bulk path on the SMF-28 8192-point forward workload within 1e-5 relative spectral energy."""
     import fiberhip
 
     problem = smf_problem(nt=8192, window_ps=10.0, policy="auto_if_undersized")
-    print("case: config0 fp32 nt=8192 steps=200", flush=True)
+    print("case: smf28-8192 fp32 nt=8192 steps=200", flush=True)
     gpu = fiberhip.scalar_rk4_output(problem, 200, precision="fp32")
     ref = _oracle_out(problem, 200)
     e_rel = mode_energy_rel(gpu, ref)
     f_rel = rel(gpu, ref)
-    print(f"config0 fp32: energy rel {e_rel}, field rel {f_rel:.3e}", flush=True)
+    print(f"smf28-8192 fp32: energy rel {e_rel}, field rel {f_rel:.3e}", flush=True)
     assert np.all(e_rel <= 1e-5)
     assert f_rel <= 1e-4
 

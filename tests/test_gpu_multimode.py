@@ -24,22 +24,22 @@ def _oracle(problem, steps, launch=None):
     return oracle.rk4_forward(oracle_fiber(problem), problem.fiber["L"], steps, u0)
 
 
-def test_config1_geometry_fp64_matches_oracle(gpu_context):
-    """Config 1 (GRIN 50 um / NA 0.2, M=6 HG modes on 256^2, Nt=4096, 12 ps,
+def test_grin6_geometry_fp64_matches_oracle(gpu_context):
+    """GRIN 50 um / NA 0.2, M=6 HG modes on 256^2, Nt=4096, 12 ps,
     10 nJ sech2 100 fs, Dirichlet launch) at the benchmark step h = 0.075 mm
     over 7.5 mm (100 steps)."""
     import fiberhip
     from fiberhip import grin
 
     problem = grin.grin_problem(6, length_m=0.0075)
-    print("case: config1 fp64 M=6 nt=4096 steps=100", flush=True)
+    print("case: grin6 fp64 M=6 nt=4096 steps=100", flush=True)
     gpu, _ = fiberhip.Plan.from_problem(problem, steps=100, precision="fp64").forward(problem.uω0)
     ref = _oracle(problem, 100)
-    print("config1 fp64 energy rel", mode_energy_rel(gpu, ref), "field rel", rel(gpu, ref), flush=True)
+    print("grin6 fp64 energy rel", mode_energy_rel(gpu, ref), "field rel", rel(gpu, ref), flush=True)
     assert np.all(mode_energy_rel(gpu, ref) <= 1e-10)
     assert rel(gpu, ref) <= 1e-10
     gpu32, _ = fiberhip.Plan.from_problem(problem, steps=100, precision="fp32").forward(problem.uω0)
-    print("config1 fp32 energy rel", mode_energy_rel(gpu32, ref), flush=True)
+    print("grin6 fp32 energy rel", mode_energy_rel(gpu32, ref), flush=True)
     assert np.all(mode_energy_rel(gpu32, ref) <= 1e-5)
 
 
@@ -121,8 +121,8 @@ def test_multimode_batch_members_equal_alone(gpu_context, precision):
         assert np.array_equal(alone, batch[b])
 
 
-def test_config1_full_fp32_vs_fp64(gpu_context):
-    """Config 1 at full length (0.3 m, 4000 steps): FP32-bulk vs FP64 per-mode
+def test_grin6_full_fp32_vs_fp64(gpu_context):
+    """GRIN M=6 at full length (0.3 m, 4000 steps): FP32-bulk vs FP64 per-mode
     spectral energy within 1e-5 (stated per mode)."""
     import fiberhip
     from fiberhip import grin
@@ -130,8 +130,8 @@ def test_config1_full_fp32_vs_fp64(gpu_context):
     problem = grin.grin_problem(6)
     out = {}
     for precision in ("fp64", "fp32"):
-        print(f"case: config1 full {precision} 4000 steps", flush=True)
+        print(f"case: grin6 full {precision} 4000 steps", flush=True)
         out[precision], _ = fiberhip.Plan.from_problem(problem, steps=4000, precision=precision).forward(problem.uω0)
     err = mode_energy_rel(out["fp32"], out["fp64"])
-    print("config1 full fp32 vs fp64 per-mode energy rel err:", err, flush=True)
+    print("grin6 full fp32 vs fp64 per-mode energy rel err:", err, flush=True)
     assert np.all(err <= 1e-5)

@@ -17,8 +17,8 @@ from fiberhip import _lib
 from oracle import np_ref as R
 
 
-def test_config0_grid_resolution():
-    """Config 0 asks for Nt=2^13 at 10 ps; the reference's auto grid policy
+def test_smf28_8192_grid_resolution():
+    """The SMF-28 forward workload asks for Nt=2^13 at 10 ps; the reference's auto grid policy
     resolves it to Nt=8192, T=23.531 ps (physics_helpers.jl:138-185)."""
     fib = fiberhip.Fiber(length_m=0.5, power_w=0.1)
     g = fiberhip.resolve_grid(fib, fiberhip.Pulse(), fiberhip.Grid(nt=8192, time_window_ps=10.0))
@@ -155,7 +155,7 @@ def test_grin_builder_invariants():
     assert np.allclose(D[:, 0], beta2 * w ** 2 / 2, rtol=1e-9, atol=1e-12)
 
 
-def test_config1_launch():
+def test_grin6_launch():
     """10 nJ sech2 100 fs -> P_peak = 0.881374 E / fwhm = 88.137 kW, split by
     seeded Dirichlet weights."""
     from fiberhip import grin

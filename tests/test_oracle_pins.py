@@ -58,7 +58,7 @@ def test_raman_response_closed_form(nt):
     assert np.allclose(hR[1:nt // 2], np.conj(hR[nt // 2 + 1:][::-1]))
 
 
-def test_grid_validator_rejects_config0_literal_window():
+def test_grid_validator_rejects_10ps_window_at_8192():
     """test_numerical_kernels.jl:45-53: Nt=8192 at 12 ps is rejected."""
     with pytest.raises(ValueError, match="use at least 21.178 ps"):
         R.sim_params(1550e-9, 1, 8192, 12.0, 2)
