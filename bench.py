@@ -25,6 +25,14 @@ sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 import numpy as np  # noqa: E402
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense vector/matrix peaks
+# HBM bytes per unit of work measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled x2.0 / x1.0 by a plain copy kernel of known size); the summaries and
+# commands are in profiles/.
+MEASURED_HBM_BYTES = {
+    "k_grad_single<float,16384,1024>": (1.4476e10, "profiles/k_grad_single_smf28_16384.md"),
+    "k_rk4ip_single<float,8192,512>": (1.4040e5, "profiles/k_rk4ip_single_smf28_8192.md"),
+    "k_rk4ip_multi<float,4096,256,6>": (4.9081e10, "profiles/k_rk4ip_multi_grin6_4096.md"),
+}
 
 
 def rhs_flops(nt: int, m: int) -> float:
@@ -272,7 +280,9 @@ def main() -> None:
                                                               / np.abs(g["fp64"]["cost"]))),
                              "max_rel_gradient_err_normwise": float(np.max(gerr))},
             "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                         "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+                         "frac": achieved / PEAK_TFLOPS["fp32"],
+                         "traffic": MEASURED_HBM_BYTES["k_grad_single<float,16384,1024>"][0] * args.batch,
+                         "traffic_source": MEASURED_HBM_BYTES["k_grad_single<float,16384,1024>"][1],
                          "kernel": "k_grad_single<float,16384,1024>", "flops_per_gradient": flops},
             "fp64_roofline": {"achieved": args.batch * flops / (g["fp64"]["kernel_ms"] / 1e3) / 1e12,
                               "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "kernel": "k_grad_single<double,16384,1024>"},
