@@ -8,8 +8,10 @@ CPU baseline), never the thing measured or shipped.  The product package
 Two independent restatements live here:
 
 * ``fiberoracle.c`` (loaded below through ctypes): forward RHS, fixed-step
-  global-frame RK4 (scalar_crosscheck.jl:8-68), its discrete adjoint and the
-  reference's continuous adjoint RHS, with a self-written radix-2 FFT.
+  global-frame RK4 (scalar_crosscheck.jl:8-68), its discrete adjoint, the
+  reference's continuous adjoint RHS, and adaptive Tsit5 (solve_disp_mmf's
+  integrator) with the discrete adjoint of its accepted-step map, with a
+  self-written radix-2 FFT.
 * ``np_ref.py``: numpy restatement of the setup helpers and the same
   RHS / RK4 / VJP, using numpy's FFT.
 
@@ -65,8 +67,15 @@ def lib() -> ctypes.CDLL:
         _LIB.orc_rk4_forward.argtypes = [fp, ctypes.c_double, ctypes.c_int, dp, dp, dp]
         _LIB.orc_rk4_adjoint.argtypes = [fp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         _LIB.orc_adjoint_rhs.argtypes = [fp, ctypes.c_double, dp, dp, dp]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        _LIB.orc_tsit5_tableau.argtypes = [dp, dp, dp]
+        _LIB.orc_tsit5_forward.argtypes = [fp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           dp, dp, dp, i64p]
+        _LIB.orc_tsit5_replay.argtypes = [fp, ctypes.c_double, dp, ctypes.c_int, dp, dp]
+        _LIB.orc_tsit5_adjoint.argtypes = [fp, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         for name in ("orc_fft", "orc_rhs", "orc_rhs_vjp", "orc_rk4_forward", "orc_rk4_adjoint",
-                     "orc_adjoint_rhs"):
+                     "orc_adjoint_rhs", "orc_tsit5_tableau", "orc_tsit5_forward", "orc_tsit5_replay",
+                     "orc_tsit5_adjoint"):
             getattr(_LIB, name).restype = ctypes.c_int
     return _LIB
 
@@ -186,6 +195,68 @@ def continuous_adjoint(fib: Fiber, length: float, steps: int, u0, lam_out) -> np
     return lam
 
 
+def tsit5_tableau():
+    """(c[7], A[7][6], btilde[7]) of the Tsitouras 5(4) pair used by the oracle."""
+    c, a, bt = np.zeros(7), np.zeros((7, 6)), np.zeros(7)
+    _check(lib().orc_tsit5_tableau(_dp(c), _dp(a), _dp(bt)), "orc_tsit5_tableau")
+    return c, a, bt
+
+
+class Tsit5Result:
+    def __init__(self, out, hs, stats):
+        self.out, self.hs, self.stats = out, hs, stats
+
+    @property
+    def accepted(self) -> int:
+        return int(self.stats[0])
+
+    @property
+    def rejected(self) -> int:
+        return int(self.stats[1])
+
+    @property
+    def rhs_evals(self) -> int:
+        return int(self.stats[2])
+
+
+def tsit5_forward(fib: Fiber, length: float, u0, rtol: float = 1e-6, atol: float = 1e-6,
+                  max_steps: int = 100000) -> Tsit5Result:
+    """Adaptive Tsit5 (solve_disp_mmf, simulate_disp_mmf.jl:183-207) -> lab spectrum,
+    accepted step sizes and {accepted, rejected, rhs evaluations}."""
+    u0 = _field(u0)
+    out = np.zeros_like(u0)
+    hs = np.zeros(max_steps)
+    st = np.zeros(3, dtype=np.int64)
+    rc = lib().orc_tsit5_forward(fib.ref, length, rtol, atol, max_steps, _dp(u0.view(np.float64)),
+                                 _dp(out.view(np.float64)), _dp(hs),
+                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    if rc == -3:
+        raise RuntimeError("orc_tsit5_forward: max_steps reached")
+    _check(rc, "orc_tsit5_forward")
+    return Tsit5Result(out.T, hs[: st[0]].copy(), st)
+
+
+def tsit5_replay(fib: Fiber, length: float, hs, u0) -> np.ndarray:
+    """The accepted-step map of tsit5_forward for a given step sequence."""
+    u0 = _field(u0)
+    out = np.zeros_like(u0)
+    hs = np.ascontiguousarray(hs, dtype=np.float64)
+    _check(lib().orc_tsit5_replay(fib.ref, length, _dp(hs), hs.size, _dp(u0.view(np.float64)),
+                                  _dp(out.view(np.float64))), "orc_tsit5_replay")
+    return out.T
+
+
+def tsit5_adjoint(fib: Fiber, length: float, hs, u0, lam_out, every: int = 50) -> np.ndarray:
+    """Discrete adjoint of tsit5_replay: Wirtinger seed at the lab output -> seed at u0."""
+    u0, lo = _field(u0), _field(lam_out)
+    out = np.zeros_like(u0)
+    hs = np.ascontiguousarray(hs, dtype=np.float64)
+    _check(lib().orc_tsit5_adjoint(fib.ref, length, _dp(hs), hs.size, every, _dp(u0.view(np.float64)),
+                                   _dp(lo.view(np.float64)), _dp(out.view(np.float64))), "orc_tsit5_adjoint")
+    return out.T
+
+
 __all__ = ["Fiber", "build", "lib", "fft", "rhs", "rhs_vjp", "rk4_forward", "rk4_adjoint",
-           "adjoint_rhs", "continuous_adjoint", "np_ref"]
+           "adjoint_rhs", "continuous_adjoint", "tsit5_tableau", "tsit5_forward", "tsit5_replay",
+           "tsit5_adjoint", "np_ref"]
 _ = os  # keep import for callers that patch paths

@@ -664,3 +664,278 @@ done:
     fft_plan_free(&p);
     return rc;
 }
+
+/* ------------------------------------------------ adaptive Tsit5 (a3) */
+
+/* Tsitouras 5(4) pair (Ch. Tsitouras, "Runge-Kutta pairs of order 5(4)
+ * satisfying only the first column simplifying assumption", Comput. Math.
+ * Appl. 62 (2011) 770-775): the Tsit5 method OrdinaryDiffEq 7.17 runs for
+ * solve_disp_mmf (simulate_disp_mmf.jl:185-195) and
+ * solve_adjoint_disp_mmf (sensitivity_disp_mmf.jl:315-329).  Row 7 of A is
+ * the 5th-order weight vector b (first same as last); BT is b - bhat, the
+ * embedded error weights. */
+static const double TS_C[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+static const double TS_A[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774}};
+static const double TS_BT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                                -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                                0.015151515151515152};
+
+int orc_tsit5_tableau(double *c, double *a, double *bt)
+{
+    memcpy(c, TS_C, sizeof(TS_C));
+    memcpy(a, TS_A, sizeof(TS_A));
+    memcpy(bt, TS_BT, sizeof(TS_BT));
+    return 0;
+}
+
+/* Step-size control, restating OrdinaryDiffEq's defaults for a 5th-order
+ * explicit pair: error norm sqrt(mean |e|^2) with e = utilde / (atol +
+ * max(|u_prev|, |u_new|) rtol) per complex entry; PI controller with
+ * beta1 = 7/(10*5), beta2 = 2/(5*5), safety 0.9, qmin 1/5, qmax 10,
+ * qold initialised to 1e-4 and set to max(EEst, 1e-4) on acceptance; a
+ * rejected step shrinks by min(1/qmin, EEst^beta1 / 0.9).  The starting step
+ * is the Hairer-Norsett-Wanner estimate (Solving ODEs I, II.4) with exponent
+ * 1/(order+1).  The constants below are shared with libfiberhip's kernels. */
+#define TS_BETA1 (7.0 / 50.0)
+#define TS_BETA2 (2.0 / 25.0)
+#define TS_SAFETY 0.9
+#define TS_QMIN 0.2
+#define TS_QMAX 10.0
+#define TS_QOLDINIT 1e-4
+
+static double scaled_norm(const cplx *e, const cplx *u0, const cplx *u1, size_t n, double rtol, double atol,
+                          double scale)
+{
+    double s = 0;
+    for (size_t i = 0; i < n; ++i) {
+        double au = cabs(u0[i]), bu = u1 ? cabs(u1[i]) : 0.0;
+        double sk = atol + (au > bu ? au : bu) * rtol;
+        double r = cabs(e[i]) * scale / sk;
+        s += r * r;
+    }
+    return sqrt(s / (double)n);
+}
+
+/* One Tsit5 attempt from (z, y, k[0] = f(z, y)) with step h: fills k[1..6]
+ * (k[6] = f(z+h, ynew)), ynew and returns EEst. */
+static double tsit5_attempt(const orc_fiber *f, orc_ws *ws, double z, double h, const cplx *y, cplx *k,
+                            cplx *ynew, cplx *tmp, double rtol, double atol)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    for (int i = 1; i < 7; ++i) {
+        cplx *dst = i == 6 ? ynew : tmp;
+        for (size_t o = 0; o < nf; ++o) {
+            cplx s = 0;
+            for (int j = 0; j < i; ++j) s += TS_A[i][j] * k[(size_t)j * nf + o];
+            dst[o] = y[o] + h * s;
+        }
+        rhs_eval(f, ws, z + TS_C[i] * h, dst, k + (size_t)i * nf);
+    }
+    for (size_t o = 0; o < nf; ++o) {
+        cplx s = 0;
+        for (int j = 0; j < 7; ++j) s += TS_BT[j] * k[(size_t)j * nf + o];
+        tmp[o] = s;
+    }
+    return scaled_norm(tmp, y, ynew, nf, rtol, atol, fabs(h));
+}
+
+/* Adaptive Tsit5 from z = 0 to L (solve_disp_mmf with reltol/abstol,
+ * simulate_disp_mmf.jl:183-207).  out = cis(D L) y(L).  hs (may be NULL,
+ * max_steps entries) receives the accepted step sizes; stats = {accepted,
+ * rejected, rhs evaluations}.  Returns 0, -1 (bad args / memory), -2
+ * (non-finite) or -3 (max_steps reached). */
+int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double atol, int max_steps,
+                      const double *u0, double *out, double *hs, int64_t *stats)
+{
+    if (length_m <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0) return -1;
+    const size_t nf = (size_t)f->nt * f->m;
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    cplx *k = malloc(sizeof(cplx) * nf * 7), *y = malloc(sizeof(cplx) * nf);
+    cplx *ynew = malloc(sizeof(cplx) * nf), *tmp = malloc(sizeof(cplx) * nf);
+    if (!k || !y || !ynew || !tmp) { free(k); free(y); free(ynew); free(tmp); ws_free(&ws); return -1; }
+    memcpy(y, u0, sizeof(cplx) * nf);
+    int64_t nacc = 0, nrej = 0, nfe = 0;
+    int rc = 0;
+    /* starting step */
+    rhs_eval(f, &ws, 0.0, y, k);
+    nfe++;
+    {
+        double d0 = scaled_norm(y, y, NULL, nf, rtol, atol, 1.0);
+        double d1 = scaled_norm(k, y, NULL, nf, rtol, atol, 1.0);
+        double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+        if (dt0 > length_m) dt0 = length_m;
+        for (size_t o = 0; o < nf; ++o) tmp[o] = y[o] + dt0 * k[o];
+        rhs_eval(f, &ws, dt0, tmp, ynew);
+        nfe++;
+        for (size_t o = 0; o < nf; ++o) tmp[o] = ynew[o] - k[o];
+        double d2 = scaled_norm(tmp, y, NULL, nf, rtol, atol, 1.0) / dt0;
+        double mx = d1 > d2 ? d1 : d2;
+        double dt1 = mx <= 1e-15 ? (dt0 * 1e-3 > 1e-6 ? dt0 * 1e-3 : 1e-6) : pow(0.01 / mx, 1.0 / 6.0);
+        double h = 100 * dt0;
+        if (dt1 < h) h = dt1;
+        if (length_m < h) h = length_m;
+        double z = 0.0, qold = TS_QOLDINIT;
+        int done = 0;
+        while (!done) {
+            if (nacc >= max_steps) { rc = -3; break; }
+            int last = 0;
+            if (z + h >= length_m) { h = length_m - z; last = 1; }
+            double e = tsit5_attempt(f, &ws, z, h, y, k, ynew, tmp, rtol, atol);
+            nfe += 6;
+            if (!isfinite(e)) { rc = -2; break; }
+            double q11 = pow(e, TS_BETA1);
+            if (e <= 1.0) {
+                double q = q11 / pow(qold, TS_BETA2);
+                q = q / TS_SAFETY;
+                if (q < 1.0 / TS_QMAX) q = 1.0 / TS_QMAX;
+                if (q > 1.0 / TS_QMIN) q = 1.0 / TS_QMIN;
+                qold = e > TS_QOLDINIT ? e : TS_QOLDINIT;
+                if (hs) hs[nacc] = h;
+                nacc++;
+                z += h;
+                memcpy(y, ynew, sizeof(cplx) * nf);
+                memcpy(k, k + 6 * nf, sizeof(cplx) * nf);
+                h = h / q;
+                done = last;
+            } else {
+                nrej++;
+                double sh = q11 / TS_SAFETY;
+                if (sh > 1.0 / TS_QMIN) sh = 1.0 / TS_QMIN;
+                h = h / sh;
+            }
+        }
+    }
+    if (stats) { stats[0] = nacc; stats[1] = nrej; stats[2] = nfe; }
+    if (rc == 0) {
+        cplx *o = (cplx *)out;
+        for (size_t i = 0; i < nf; ++i) {
+            o[i] = cis(f->disp[i] * length_m) * y[i];
+            if (!isfinite(creal(o[i])) || !isfinite(cimag(o[i]))) rc = -2;
+        }
+    }
+    free(k); free(y); free(ynew); free(tmp);
+    ws_free(&ws);
+    return rc;
+}
+
+/* One Tsit5 step with a given h (no error control), 6 stages; b = row 7 of A.
+ * If ys != NULL the stage inputs Y_1..Y_6 are stored there. */
+static void tsit5_step_fixed(const orc_fiber *f, orc_ws *ws, double z, double h, cplx *y, cplx *k, cplx *tmp,
+                             cplx *ys)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    if (ys) memcpy(ys, y, sizeof(cplx) * nf);
+    rhs_eval(f, ws, z, y, k);
+    for (int i = 1; i < 6; ++i) {
+        cplx *dst = ys ? ys + (size_t)i * nf : tmp;
+        for (size_t o = 0; o < nf; ++o) {
+            cplx s = 0;
+            for (int j = 0; j < i; ++j) s += TS_A[i][j] * k[(size_t)j * nf + o];
+            dst[o] = y[o] + h * s;
+        }
+        rhs_eval(f, ws, z + TS_C[i] * h, dst, k + (size_t)i * nf);
+    }
+    for (size_t o = 0; o < nf; ++o) {
+        cplx s = 0;
+        for (int j = 0; j < 6; ++j) s += TS_A[6][j] * k[(size_t)j * nf + o];
+        y[o] += h * s;
+    }
+}
+
+/* Forward over a recorded step sequence hs[0..n-1] (z accumulated as in
+ * orc_tsit5_forward); out = cis(D L) y, L = length_m. */
+int orc_tsit5_replay(const orc_fiber *f, double length_m, const double *hs, int n, const double *u0, double *out)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    orc_ws ws;
+    if (n <= 0 || ws_init(&ws, f->nt, f->m)) return -1;
+    cplx *k = malloc(sizeof(cplx) * nf * 6), *y = malloc(sizeof(cplx) * nf), *tmp = malloc(sizeof(cplx) * nf);
+    if (!k || !y || !tmp) { free(k); free(y); free(tmp); ws_free(&ws); return -1; }
+    memcpy(y, u0, sizeof(cplx) * nf);
+    double z = 0.0;
+    for (int s = 0; s < n; ++s) {
+        tsit5_step_fixed(f, &ws, z, hs[s], y, k, tmp, NULL);
+        z += hs[s];
+    }
+    cplx *o = (cplx *)out;
+    int rc = 0;
+    for (size_t i = 0; i < nf; ++i) {
+        o[i] = cis(f->disp[i] * length_m) * y[i];
+        if (!isfinite(creal(o[i])) || !isfinite(cimag(o[i]))) rc = -2;
+    }
+    free(k); free(y); free(tmp);
+    ws_free(&ws);
+    return rc;
+}
+
+/* Discrete adjoint of orc_tsit5_replay (the accepted-step map of the
+ * adaptive solve, step sizes held fixed).  Stage cotangents, for i = 6..1:
+ *   gk_i = h b_i lam' + h sum_{l>i} a_li gY_l,  gY_i = VJP_f(z + c_i h, Y_i)(gk_i)
+ * and lam = lam' + sum_i gY_i.  Checkpoints every `every` steps. */
+int orc_tsit5_adjoint(const orc_fiber *f, double length_m, const double *hs, int n, int every, const double *u0,
+                      const double *lam_out, double *lam0)
+{
+    if (n <= 0 || every <= 0) return -1;
+    const size_t nf = (size_t)f->nt * f->m;
+    const int nseg = (n + every - 1) / every;
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, f->m)) return -1;
+    cplx *ck = malloc(sizeof(cplx) * nf * (size_t)nseg), *seg = malloc(sizeof(cplx) * nf * (size_t)every);
+    cplx *k = malloc(sizeof(cplx) * nf * 6), *ys = malloc(sizeof(cplx) * nf * 6), *gy = malloc(sizeof(cplx) * nf * 6);
+    cplx *y = malloc(sizeof(cplx) * nf), *lam = malloc(sizeof(cplx) * nf), *tmp = malloc(sizeof(cplx) * nf);
+    cplx *gk = malloc(sizeof(cplx) * nf);
+    double *zs = malloc(sizeof(double) * (size_t)n);
+    int rc = 0;
+    if (!ck || !seg || !k || !ys || !gy || !y || !lam || !tmp || !gk || !zs) { rc = -1; goto out; }
+    memcpy(y, u0, sizeof(cplx) * nf);
+    double z = 0.0;
+    for (int s = 0; s < n; ++s) {
+        zs[s] = z;
+        if (s % every == 0) memcpy(ck + nf * (size_t)(s / every), y, sizeof(cplx) * nf);
+        tsit5_step_fixed(f, &ws, z, hs[s], y, k, tmp, NULL);
+        z += hs[s];
+    }
+    const cplx *lo = (const cplx *)lam_out;
+    for (size_t i = 0; i < nf; ++i) lam[i] = cis(-f->disp[i] * length_m) * lo[i];
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * every, s1 = (s0 + every < n) ? s0 + every : n;
+        memcpy(y, ck + nf * (size_t)sg, sizeof(cplx) * nf);
+        for (int s = s0; s < s1; ++s) {
+            memcpy(seg + nf * (size_t)(s - s0), y, sizeof(cplx) * nf);
+            if (s + 1 < s1) tsit5_step_fixed(f, &ws, zs[s], hs[s], y, k, tmp, NULL);
+        }
+        for (int s = s1 - 1; s >= s0; --s) {
+            const double h = hs[s];
+            memcpy(y, seg + nf * (size_t)(s - s0), sizeof(cplx) * nf);
+            tsit5_step_fixed(f, &ws, zs[s], h, y, k, tmp, ys);
+            for (int i = 5; i >= 0; --i) {
+                for (size_t o = 0; o < nf; ++o) {
+                    cplx g = h * TS_A[6][i] * lam[o];
+                    for (int l = i + 1; l < 6; ++l) g += h * TS_A[l][i] * gy[(size_t)l * nf + o];
+                    gk[o] = g;
+                }
+                rhs_vjp(f, &ws, zs[s] + TS_C[i] * h, ys + (size_t)i * nf, gk, gy + (size_t)i * nf);
+            }
+            for (size_t o = 0; o < nf; ++o) {
+                cplx acc = lam[o];
+                for (int i = 0; i < 6; ++i) acc += gy[(size_t)i * nf + o];
+                lam[o] = acc;
+            }
+        }
+    }
+    memcpy(lam0, lam, sizeof(cplx) * nf);
+    for (size_t i = 0; i < nf; ++i)
+        if (!isfinite(creal(lam[i])) || !isfinite(cimag(lam[i]))) rc = -2;
+out:
+    free(ck); free(seg); free(k); free(ys); free(gy); free(y); free(lam); free(tmp); free(gk); free(zs);
+    ws_free(&ws);
+    return rc;
+}
