@@ -14,6 +14,12 @@ template <typename R, int N> struct SingleShape {
                                             : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4));
 };
 
+template <typename R> struct TsArgs;
+bool tsit5_supported(int nt, int m);
+int tsit5_fields(int nt, bool fp64);
+hipError_t launch_tsit5(int nt, bool grad, const TsArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hipStream_t s);
+
 bool multi_supported(int nt, int m, bool fp64);
 int multi_fields(int m);
 hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);

@@ -13,6 +13,7 @@
 #include "fh_launch.hpp"
 #include "fh_multi.hpp"
 #include "fh_single.hpp"
+#include "fh_tsit5.hpp"
 
 namespace {
 
@@ -57,6 +58,9 @@ struct fh_plan_s {
     size_t work_per_solve = 0;  // complex entries
     int64_t total_bytes = 0;
     double gamma0 = 0.0;        // gamma[1,1,1,1] of a single-mode plan
+    bool adaptive = false;      // FH_SCHEME_TSIT5
+    int max_steps = 0;
+    DevBuf hs, zs, sstats;      // adaptive: step sizes, positions [max_batch][max_steps]; counters [max_batch][3]
 };
 
 namespace {
@@ -200,8 +204,15 @@ int check_desc(const fh_problem_desc* d)
     if (d->modes < 1) return fail(FH_EINVAL, "modes must be positive");
     if (d->precision != FH_PREC_FP64 && d->precision != FH_PREC_FP32_BULK)
         return fail(FH_EINVAL, "unknown precision");
-    if (d->scheme != FH_SCHEME_RK4IP_FIXED) return fail(FH_EINVAL, "unknown scheme");
-    if (d->steps <= 0) return fail(FH_EINVAL, "steps must be a positive integer");
+    if (d->scheme != FH_SCHEME_RK4IP_FIXED && d->scheme != FH_SCHEME_TSIT5) return fail(FH_EINVAL, "unknown scheme");
+    if (d->scheme == FH_SCHEME_RK4IP_FIXED && d->steps <= 0) return fail(FH_EINVAL, "steps must be a positive integer");
+    if (d->scheme == FH_SCHEME_TSIT5) {
+        if (!(std::isfinite(d->rtol) && d->rtol > 0)) return fail(FH_EINVAL, "rtol must be positive and finite");
+        if (!(std::isfinite(d->atol) && d->atol >= 0)) return fail(FH_EINVAL, "atol must be nonnegative and finite");
+        if (d->max_steps < 0) return fail(FH_EINVAL, "max_steps must be >= 0");
+        if (!fh::tsit5_supported(d->nt, d->modes))
+            return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..16384 samples");
+    }
     if (d->max_batch <= 0) return fail(FH_EINVAL, "max_batch must be positive");
     if (!(std::isfinite(d->length_m) && d->length_m > 0)) return fail(FH_EINVAL, "fiber length must be positive and finite");
     if (!d->dispersion || !d->gamma || !d->raman || !d->shock) return fail(FH_EINVAL, "operator array is NULL");
@@ -271,9 +282,16 @@ template <typename R> int upload_tables(fh_plan p)
         p->has_mask = true;
     }
     p->every = d.checkpoint_every > 0 ? d.checkpoint_every : 50;
-    if (p->every > d.steps) p->every = d.steps;
-    p->nseg = (d.steps + p->every - 1) / p->every;
-    p->work_per_solve = d.modes == 1 ? with_size<R, OpWork>(nt) : (size_t)fh::multi_fields(d.modes) * nt;
+    if (p->adaptive) {
+        p->max_steps = d.max_steps > 0 ? d.max_steps : 10000;
+        if (p->every > p->max_steps) p->every = p->max_steps;
+        p->nseg = (p->max_steps + p->every - 1) / p->every;
+        p->work_per_solve = (size_t)fh::tsit5_fields(nt, sizeof(R) == 8) * nt;
+    } else {
+        if (p->every > d.steps) p->every = d.steps;
+        p->nseg = (d.steps + p->every - 1) / p->every;
+        p->work_per_solve = d.modes == 1 ? with_size<R, OpWork>(nt) : (size_t)fh::multi_fields(d.modes) * nt;
+    }
     if (d.modes > 1) {
         const int M = d.modes;
         auto G = [&](int i, int j, int k, int l) { return d.gamma[i + M * (j + M * (k + M * l))]; };
@@ -326,7 +344,25 @@ template <typename R> int upload_tables(fh_plan p)
     const size_t field = 2 * sizeof(R) * (size_t)nt * d.modes * mb;
     if ((rc = dev_alloc(p, p->u_in, field))) return rc;
     if ((rc = dev_alloc(p, p->u_out, field))) return rc;
+    if (p->adaptive) {
+        if ((rc = dev_alloc(p, p->hs, sizeof(double) * mb * (size_t)p->max_steps))) return rc;
+        if ((rc = dev_alloc(p, p->zs, sizeof(double) * mb * (size_t)p->max_steps))) return rc;
+        if ((rc = dev_alloc(p, p->sstats, sizeof(long long) * 3 * mb))) return rc;
+    }
     return FH_OK;
+}
+
+template <typename R> fh::TsArgs<R> ts_args(fh_plan p, const fh::SmArgs<R>& s)
+{
+    fh::TsArgs<R> a{};
+    a.s = s;
+    a.rtol = p->d.rtol;
+    a.atol = p->d.atol;
+    a.max_steps = p->max_steps;
+    a.hs = (double*)p->hs.p;
+    a.zs = (double*)p->zs.p;
+    a.stats = (long long*)p->sstats.p;
+    return a;
 }
 
 template <typename R> fh::SmArgs<R> base_args(fh_plan p)
@@ -357,8 +393,9 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
     const size_t nt = (size_t)p->d.nt, mb = (size_t)p->d.max_batch, V = 2 * sizeof(R);
     int rc;
     if ((rc = dev_alloc(p, p->ckpt, V * nt * (size_t)p->nseg * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_u, V * nt * 4 * (size_t)p->every * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_h, sizeof(R) * nt * 4 * (size_t)p->every * mb))) return rc;
+    const size_t stages = p->adaptive ? 6 : 4;
+    if ((rc = dev_alloc(p, p->tape_u, V * nt * stages * (size_t)p->every * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, sizeof(R) * nt * stages * (size_t)p->every * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_lam0, V * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;
@@ -386,7 +423,8 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
     a.tape_u = (fh::C2<R>*)p->tape_u.p;
     a.tape_h = (R*)p->tape_h.p;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
-    hipError_t e = with_size<R, OpGrad>(p->d.nt, a, batch, s);
+    hipError_t e = p->adaptive ? fh::launch_tsit5(p->d.nt, true, ts_args<R>(p, a), batch, s)
+                               : with_size<R, OpGrad>(p->d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("gradient launch: ") + hipGetErrorString(e));
     return FH_OK;
 }
@@ -424,8 +462,51 @@ template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0,
     a.u0 = (const fh::C2<R>*)d_u0;
     a.out = (fh::C2<R>*)d_uL;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
-    hipError_t e = with_size<R, OpLaunch>(d.nt, a, batch, s);
+    hipError_t e = p->adaptive ? fh::launch_tsit5(d.nt, false, ts_args<R>(p, a), batch, s)
+                               : with_size<R, OpLaunch>(d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return FH_OK;
+}
+
+// Per-solve counters of the last call (device counters for the adaptive
+// scheme, the fixed step count otherwise).
+int read_solve_stats(fh_plan p, int batch, std::vector<long long>& st)
+{
+    st.assign(3 * (size_t)batch, 0);
+    if (p->adaptive) {
+        FH_HIP(hipMemcpy(st.data(), p->sstats.p, sizeof(long long) * 3 * (size_t)batch, hipMemcpyDeviceToHost));
+    } else {
+        for (int b = 0; b < batch; ++b) {
+            st[3 * b] = p->d.steps;
+            st[3 * b + 2] = 4LL * p->d.steps;
+        }
+    }
+    return FH_OK;
+}
+
+int finish_call(fh_plan p, int batch, const std::vector<int>& status, fh_stats* stats, bool grad)
+{
+    int bad = 0;
+    for (int b = 0; b < batch; ++b) bad |= status[b];
+    std::vector<long long> st;
+    int rc = read_solve_stats(p, batch, st);
+    if (rc) return rc;
+    const int code = (bad & 2) ? FH_EMAXSTEPS : (bad ? FH_ENONFINITE : FH_OK);
+    if (stats) {
+        stats->accepted = stats->rejected = stats->rhs_evals = 0;
+        for (int b = 0; b < batch; ++b) {
+            stats->accepted += st[3 * b];
+            stats->rejected += st[3 * b + 1];
+            stats->rhs_evals += st[3 * b + 2];
+        }
+        stats->retcode = code;
+    }
+    if (code == FH_EMAXSTEPS)
+        return fail(code, "adaptive propagation reached max_steps=" + std::to_string(p->max_steps) +
+                              " accepted steps before the fiber end");
+    if (code)
+        return fail(code, grad ? "adjoint propagation produced non-finite fields"
+                               : "propagation produced non-finite fields");
     return FH_OK;
 }
 
@@ -472,6 +553,7 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     p->ctx = ctx;
     p->d = *desc;  // array pointers are only read during creation
     p->gamma0 = desc->gamma[0];
+    p->adaptive = desc->scheme == FH_SCHEME_TSIT5;
     rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p) : upload_tables<float>(p);
     p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
     p->d.band_mask = nullptr;
@@ -488,7 +570,7 @@ int fh_plan_destroy(fh_plan p)
     if (!p) return FH_OK;
     (void)hipSetDevice(p->ctx->device);
     for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out, &p->mask,
-                      &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed, &p->gr, &p->qidx})
+                      &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed, &p->gr, &p->qidx, &p->hs, &p->zs, &p->sstats})
         dev_free(*b);
     delete p;
     return FH_OK;
@@ -549,16 +631,7 @@ int fh_forward(fh_plan p, int32_t batch, const double* u0, double* uL, fh_stats*
         FH_HIP(hipStreamSynchronize(s));
         for (size_t i = 0; i < 2 * n; ++i) uL[i] = tmp[i];
     }
-    int bad = 0;
-    for (int b = 0; b < batch; ++b) bad |= status[b];
-    if (stats) {
-        stats->accepted = p->d.steps;
-        stats->rejected = 0;
-        stats->rhs_evals = 4LL * p->d.steps;
-        stats->retcode = bad ? FH_ENONFINITE : FH_OK;
-    }
-    if (bad) return fail(FH_ENONFINITE, "fixed-step propagation produced non-finite fields");
-    return FH_OK;
+    return finish_call(p, batch, status, stats, false);
 }
 
 int fh_device_alloc(fh_context ctx, int64_t bytes, void** ptr)
@@ -744,15 +817,28 @@ int fh_gradient(fh_plan p, int32_t batch, const double* u0, const double* seed, 
         else
             FH_HIP(hipMemcpy(cost, p->g_cost.p, sizeof(double) * batch, hipMemcpyDeviceToHost));
     }
-    int bad = 0;
-    for (int b = 0; b < batch; ++b) bad |= status[b];
-    if (stats) {
-        stats->accepted = 2LL * p->d.steps;
-        stats->rejected = 0;
-        stats->rhs_evals = 8LL * p->d.steps;
-        stats->retcode = bad ? FH_ENONFINITE : FH_OK;
-    }
-    if (bad) return fail(FH_ENONFINITE, "adjoint propagation produced non-finite fields");
+    return finish_call(p, batch, status, stats, true);
+}
+
+int fh_solve_stats(fh_plan p, int32_t batch, int64_t* stats)
+{
+    if (!p || !stats) return fail(FH_EINVAL, "plan or stats is NULL");
+    if (batch < 1 || batch > p->d.max_batch) return fail(FH_EINVAL, "batch must lie in [1, max_batch]");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    std::vector<long long> st;
+    int rc = read_solve_stats(p, batch, st);
+    if (rc) return rc;
+    for (size_t i = 0; i < st.size(); ++i) stats[i] = st[i];
+    return FH_OK;
+}
+
+int fh_step_sizes(fh_plan p, int32_t batch, double* hs)
+{
+    if (!p || !hs) return fail(FH_EINVAL, "plan or hs is NULL");
+    if (!p->adaptive) return fail(FH_EINVAL, "step sizes are recorded by the adaptive scheme only");
+    if (batch < 1 || batch > p->d.max_batch) return fail(FH_EINVAL, "batch must lie in [1, max_batch]");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    FH_HIP(hipMemcpy(hs, p->hs.p, sizeof(double) * (size_t)batch * p->max_steps, hipMemcpyDeviceToHost));
     return FH_OK;
 }
 

@@ -21,15 +21,17 @@ FH_ENONFINITE = 4
 FH_ENOMEM = 5
 FH_EDEVICE = 6
 FH_EUNSUPPORTED = 7
+FH_EMAXSTEPS = 8
 
 FH_PREC_FP64 = 0
 FH_PREC_FP32_BULK = 1
 FH_SCHEME_RK4IP_FIXED = 0
+FH_SCHEME_TSIT5 = 1
 
 _NAMES = {
     FH_EINVAL: "FH_EINVAL", FH_EGRID: "FH_EGRID", FH_ENOTSYM: "FH_ENOTSYM",
     FH_ENONFINITE: "FH_ENONFINITE", FH_ENOMEM: "FH_ENOMEM", FH_EDEVICE: "FH_EDEVICE",
-    FH_EUNSUPPORTED: "FH_EUNSUPPORTED",
+    FH_EUNSUPPORTED: "FH_EUNSUPPORTED", FH_EMAXSTEPS: "FH_EMAXSTEPS",
 }
 
 
@@ -63,6 +65,10 @@ class ProblemDesc(ctypes.Structure):
         ("checkpoint_every", ctypes.c_int32),
         ("log_cost", ctypes.c_int32),
         ("band_mask", ctypes.POINTER(ctypes.c_uint8)),
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("max_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -101,6 +107,8 @@ _SIGNATURES = {
     "fh_device_copy": ([_VP, _VP, _VP, ctypes.c_int64], ctypes.c_int),
     "fh_time_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)],
                                ctypes.c_int),
+    "fh_solve_stats": ([_VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_step_sizes": ([_VP, ctypes.c_int32, _DP], ctypes.c_int),
     "fh_last_error": ([], ctypes.c_char_p),
 }
 

@@ -7,9 +7,10 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import FH_PREC_FP32_BULK, FH_PREC_FP64, FH_SCHEME_RK4IP_FIXED, check, lib
+from ._lib import FH_PREC_FP32_BULK, FH_PREC_FP64, FH_SCHEME_RK4IP_FIXED, FH_SCHEME_TSIT5, check, lib
 
 PRECISIONS = {"fp64": FH_PREC_FP64, "fp32": FH_PREC_FP32_BULK, "fp32_bulk": FH_PREC_FP32_BULK}
+SCHEMES = {"rk4ip": FH_SCHEME_RK4IP_FIXED, "tsit5": FH_SCHEME_TSIT5}
 
 
 def _dp(a: np.ndarray):
@@ -97,16 +98,24 @@ class Plan:
     shock (nt,) = fftshift(ωs/ω0), all in FFT order as the reference builds them.
     """
 
-    def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: int,
+    def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: Optional[int],
                  precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None,
-                 band_mask=None, checkpoint_every: int = 50, log_cost: bool = False):
+                 band_mask=None, checkpoint_every: int = 50, log_cost: bool = False, scheme: Optional[str] = None,
+                 rtol: float = 1e-8, atol: float = 1e-6, max_steps: int = 10000):
         self.context = context or default_context()
         D = np.asarray(dispersion, dtype=np.float64)
         if D.ndim == 1:
             D = D[:, None]
         self.nt, self.modes = D.shape
         self.precision = precision
-        self.steps = int(steps)
+        if scheme is None:
+            scheme = "rk4ip" if steps else "tsit5"
+        if scheme not in SCHEMES:
+            raise ValueError(f"scheme must be one of {sorted(SCHEMES)}")
+        self.scheme = scheme
+        self.steps = int(steps or 0)
+        self.max_steps = int(max_steps)
+        self.rtol, self.atol = float(rtol), float(atol)
         self.length_m = float(length_m)
         self.max_batch = int(max_batch)
         self._D = _column_major(D, np.float64)
@@ -117,26 +126,47 @@ class Plan:
         self._mask = None if band_mask is None else np.ascontiguousarray(np.asarray(band_mask, dtype=np.uint8))
         self.has_band = band_mask is not None
         desc = _lib.ProblemDesc(
-            nt=self.nt, modes=self.modes, precision=PRECISIONS[precision], scheme=FH_SCHEME_RK4IP_FIXED,
+            nt=self.nt, modes=self.modes, precision=PRECISIONS[precision], scheme=SCHEMES[scheme],
             steps=self.steps, max_batch=self.max_batch, length_m=self.length_m,
             dispersion=_dp(self._D), gamma=_dp(self._g), raman=_dp(self._hr.view(np.float64)),
             shock=_dp(self._sh), one_m_fr=float(one_m_fr), checkpoint_every=int(checkpoint_every),
             log_cost=int(bool(log_cost)),
-            band_mask=None if self._mask is None else self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+            band_mask=None if self._mask is None else self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            rtol=self.rtol, atol=self.atol, max_steps=self.max_steps)
         h = ctypes.c_void_p()
         check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
 
     @classmethod
-    def from_problem(cls, problem, steps: int, precision: str = "fp64", max_batch: int = 1,
+    def from_problem(cls, problem, steps: Optional[int] = None, precision: str = "fp64", max_batch: int = 1,
                      context: Optional[Context] = None, band_mask=None, checkpoint_every: int = 50,
-                     log_cost: bool = False) -> "Plan":
+                     log_cost: bool = False, scheme: Optional[str] = None, rtol: Optional[float] = None,
+                     atol: Optional[float] = None, max_steps: int = 10000) -> "Plan":
+        """Plan for a FiberFieldProblem.  With `steps` the fixed-step RK4IP of
+        _scalar_rk4_output; without, the adaptive Tsit5 of solve_disp_mmf with the
+        fiber's reltol/abstol (defaults 1e-8 / 1e-6, simulate_disp_mmf.jl:186-190)."""
         f = problem.fiber
         from .problem import self_steepening
 
+        rtol = f.get("reltol", 1e-8) if rtol is None else rtol
+        atol = f.get("abstol", 1e-6) if atol is None else atol
         return cls(f["Dω"], f["γ"], f["hRω"], self_steepening(problem.sim), f["one_m_fR"], f["L"], steps,
                    precision=precision, max_batch=max_batch, context=context, band_mask=band_mask,
-                   checkpoint_every=checkpoint_every, log_cost=log_cost)
+                   checkpoint_every=checkpoint_every, log_cost=log_cost, scheme=scheme, rtol=rtol, atol=atol,
+                   max_steps=max_steps)
+
+    def solve_stats(self, batch: int) -> np.ndarray:
+        """(batch, 3) accepted, rejected, rhs evaluations of the last call (fh_solve_stats)."""
+        out = np.zeros((batch, 3), dtype=np.int64)
+        check(lib().fh_solve_stats(self.handle, int(batch), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def step_sizes(self, batch: int) -> list:
+        """Accepted step sizes of each solve of the last adaptive call (fh_step_sizes)."""
+        hs = np.zeros((batch, self.max_steps))
+        check(lib().fh_step_sizes(self.handle, int(batch), _dp(hs)))
+        n = self.solve_stats(batch)[:, 0]
+        return [hs[b, : n[b]].copy() for b in range(batch)]
 
     @property
     def device_bytes(self) -> int:

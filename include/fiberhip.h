@@ -40,7 +40,9 @@ extern "C" {
  *                 propagation.jl:48-53)
  *   FH_ENOMEM     device allocation failed
  *   FH_EDEVICE    HIP runtime error
- *   FH_EUNSUPPORTED a size or mode count this build has no kernel for */
+ *   FH_EUNSUPPORTED a size or mode count this build has no kernel for
+ *   FH_EMAXSTEPS  adaptive solve hit max_steps accepted steps (the
+ *                 MaxIters retcode that propagation.jl:48-53 rejects) */
 enum {
     FH_OK = 0,
     FH_EINVAL = 1,
@@ -49,7 +51,8 @@ enum {
     FH_ENONFINITE = 4,
     FH_ENOMEM = 5,
     FH_EDEVICE = 6,
-    FH_EUNSUPPORTED = 7
+    FH_EUNSUPPORTED = 7,
+    FH_EMAXSTEPS = 8
 };
 
 /* Arithmetic of the bulk field work.  FH_PREC_FP32_BULK keeps fields, FFTs
@@ -58,8 +61,12 @@ enum {
 enum { FH_PREC_FP64 = 0, FH_PREC_FP32_BULK = 1 };
 
 /* Integrator.  FH_SCHEME_RK4IP_FIXED is the global-frame interaction-picture
- * RK4 of _scalar_rk4_output (scalar_crosscheck.jl:53-60), z_i = i*h. */
-enum { FH_SCHEME_RK4IP_FIXED = 0 };
+ * RK4 of _scalar_rk4_output (scalar_crosscheck.jl:53-60), z_i = i*h, with
+ * `steps` steps.  FH_SCHEME_TSIT5 is the adaptive Tsit5 (Tsitouras 5(4)) of
+ * solve_disp_mmf (simulate_disp_mmf.jl:183-207) with relative/absolute
+ * tolerances rtol/atol, an FP64 RMS error norm and PI step control; every
+ * solve of a batch runs its own step sequence (single mode). */
+enum { FH_SCHEME_RK4IP_FIXED = 0, FH_SCHEME_TSIT5 = 1 };
 
 typedef struct fh_context_s *fh_context;
 typedef struct fh_plan_s *fh_plan;
@@ -81,8 +88,13 @@ typedef struct {
     int32_t checkpoint_every; /* adjoint: forward states kept every K steps (0 = 50) */
     int32_t log_cost;         /* band objective in dB (_maybe_log_cost)      */
     const uint8_t *band_mask; /* nt bins of the band objective, or NULL      */
+    double rtol;              /* FH_SCHEME_TSIT5: relative tolerance (reltol) */
+    double atol;              /* FH_SCHEME_TSIT5: absolute tolerance (abstol) */
+    int32_t max_steps;        /* FH_SCHEME_TSIT5: accepted-step limit (0 = 10000) */
+    int32_t reserved;
 } fh_problem_desc;
 
+/* Counters of one call, summed over its batch (per-solve values: fh_solve_stats). */
 typedef struct {
     int64_t accepted;   /* accepted steps (sol.destats.naccept)            */
     int64_t rejected;   /* rejected steps                                  */
@@ -167,6 +179,17 @@ int fh_device_copy(fh_context ctx, void *dst, const void *src, int64_t bytes);
  * stream, wait for them, and return each launch's duration in ms_out[i]. */
 int fh_time_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, int32_t launches,
                            float *ms_out);
+
+/* Per-solve counters of the last fh_forward* / fh_gradient* call on this
+ * plan (after it completed): stats[3*b + {0,1,2}] = accepted steps, rejected
+ * steps, right-hand-side evaluations of solve b (sol.destats.naccept /
+ * .nreject / .nf).  For the fixed-step scheme these are steps, 0, 4*steps
+ * (the forward sweep of a gradient). */
+int fh_solve_stats(fh_plan plan, int32_t batch, int64_t *stats);
+
+/* Accepted step sizes of the last adaptive call on this plan:
+ * hs[b*max_steps + i], i < accepted steps of solve b (FH_SCHEME_TSIT5). */
+int fh_step_sizes(fh_plan plan, int32_t batch, double *hs);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *fh_last_error(void);

@@ -85,7 +85,8 @@ def test_batch_members_equal_single_solves(gpu_context, precision):
     plan = fiberhip.Plan.from_problem(problem, steps=40, precision=precision, max_batch=8)
     print(f"case: batch independence {precision}", flush=True)
     batch_out, stats = plan.forward(np.stack(launches))
-    assert stats["rhs_evaluations"] == 160
+    assert stats["rhs_evaluations"] == 5 * 160  # summed over the batch
+    assert np.array_equal(plan.solve_stats(5), np.tile([40, 0, 160], (5, 1)))
     for i, l in enumerate(launches):
         alone, _ = plan.forward(l)
         assert np.array_equal(alone, batch_out[i])
