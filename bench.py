@@ -69,6 +69,11 @@ def workload(name: str):
         from fiberhip import grin
 
         return grin.grin_problem(6), 4000
+    if name == "smf28-sweep-8192":
+        fib = fiberhip.Fiber(length_m=1.0, power_w=0.1, preset="SMF28", beta_order=3)
+        problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=8192, time_window_ps=23.531, policy="exact"),
+                                         raman_threshold_thz=None)
+        return problem, 0
     if name == "smf28-grad-16384":
         fib = fiberhip.Fiber(length_m=2.0, power_w=0.2, preset="SMF28", beta_order=3)
         problem = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=16384, time_window_ps=47.061, policy="exact"),
@@ -136,6 +141,37 @@ def time_gradient(problem, steps, precision, batch, k, warmup, mask):
     return dict(kernel_ms=float(np.mean(ms)), ms=ms, energies=energies,
                 grad=grad.to_host(np.float64, (batch, problem.nt)), cost=cost.to_host(np.float64, (batch,)),
                 device_bytes=plan.device_bytes)
+
+
+def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol=1e-6, atol=1e-6):
+    """Adaptive candidate sweep: `candidates` forward + adjoint gradients of the band
+    objective, launched `per_launch` at a time back to back on one stream; each
+    candidate runs its own Tsit5 step sequence.  Returns the per-sweep time and
+    the per-candidate step statistics of the last sweep."""
+    import fiberhip
+
+    plan = fiberhip.Plan.from_problem(problem, scheme="tsit5", rtol=rtol, atol=atol, precision=precision,
+                                      max_batch=per_launch, band_mask=mask, checkpoint_every=50, max_steps=4000)
+    host = launches(problem, candidates, seed=4)[:, :, None]
+    nl = candidates // per_launch
+    u0 = [plan.to_device(host[i * per_launch:(i + 1) * per_launch]) for i in range(nl)]
+    out = fiberhip.DeviceArray(plan.context, u0[0].nbytes)
+    lam0 = fiberhip.DeviceArray(plan.context, u0[0].nbytes)
+    grad = fiberhip.DeviceArray(plan.context, 8 * per_launch * problem.nt)
+    cost = fiberhip.DeviceArray(plan.context, 8 * per_launch)
+    for _ in range(warmup):
+        for i in range(nl):
+            plan.time_gradient_device(per_launch, u0[i].ptr, out.ptr, lam0.ptr, grad.ptr, cost.ptr, 1)
+    sweeps, stats, costs = [], [], []
+    for _ in range(k):
+        t = 0.0
+        for i in range(nl):
+            t += float(plan.time_gradient_device(per_launch, u0[i].ptr, out.ptr, lam0.ptr, grad.ptr, cost.ptr, 1)[0])
+            if len(stats) < nl:
+                stats.append(plan.solve_stats(per_launch))
+                costs.append(cost.to_host(np.float64, (per_launch,)))
+        sweeps.append(t)
+    return dict(ms=float(np.mean(sweeps)), stats=np.concatenate(stats), cost=np.concatenate(costs))
 
 
 def cpu_baseline_gradient(problem, steps, mask, slice_steps=200):
@@ -220,7 +256,8 @@ def main() -> None:
     ap.add_argument("--batch", type=int, default=256, help="gradients per launch (SMF-28 2 m gradient workload)")
     ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (SMF-28 0.5 m forward workload)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode"], default="all")
+    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep"], default="all")
+    ap.add_argument("--sweep-candidates", type=int, default=256, help="adaptive candidate sweep size")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
@@ -347,6 +384,45 @@ def main() -> None:
                     "data": "synthetic", "config": {"workload": mm["workload"]}, "multimode": mm}
         else:
             line["forward_grin6_4096"] = mm
+    if args.only in ("all", "sweep"):
+        problem, _ = workload("smf28-sweep-8192")
+        nt = problem.nt
+        mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+        nc = args.sweep_candidates
+        # schedule autotuning: candidates per launch, fastest FP32 sweep wins
+        tune = {}
+        for per in (nc, nc // 2, nc // 4):
+            if per >= 1 and nc % per == 0:
+                tune[per] = time_sweep(problem, "fp32", nc, per, 1, 1, mask)["ms"]
+                print(f"bench: sweep schedule {nc // per} x {per}: {tune[per]:.1f} ms", file=sys.stderr, flush=True)
+        per = min(tune, key=tune.get)
+        res = {p: time_sweep(problem, p, nc, per, args.steps, args.warmup, mask) for p in ("fp32", "fp64")}
+        st = res["fp64"]["stats"]
+        # algorithmic flops from the actual evaluation counts: forward RHS (accepted +
+        # rejected + starting step) and 6 stage pullbacks per accepted step
+        flops = float(np.sum(st[:, 2]) * rhs_flops(nt, 1) + np.sum(6 * st[:, 0]) * adjoint_rhs_flops(nt, 1)) / nc
+        ms32, ms64 = res["fp32"]["ms"], res["fp64"]["ms"]
+        sweep = {
+            "workload": f"candidate sweep: SMF-28 1 m, 0.1 W, Nt={nt}, T={problem.sim['time_window']} ps, {nc} "
+                        f"GDD/TOD/FOD candidates, adaptive Tsit5 rtol=atol=1e-6, forward + adjoint gradient of the "
+                        f"-16..-10 THz band objective",
+            "candidates": nc, "schedule": f"{nc // per} launches x {per} candidates", "autotune_ms": tune,
+            "fp32_grads_per_s": nc / (ms32 / 1e3), "fp64_grads_per_s": nc / (ms64 / 1e3),
+            "fp32_over_fp64": ms64 / ms32,
+            "fp64_accepted_steps_mean": float(np.mean(st[:, 0])), "fp64_rejected_steps_mean": float(np.mean(st[:, 1])),
+            "fp32_accepted_steps_mean": float(np.mean(res["fp32"]["stats"][:, 0])),
+            "max_rel_cost_err_fp32_vs_fp64": float(np.max(np.abs(res["fp32"]["cost"] / res["fp64"]["cost"] - 1))),
+            "flops_per_gradient": flops,
+            "fp32_roofline_frac": nc * flops / (ms32 / 1e3) / 1e12 / PEAK_TFLOPS["fp32"],
+            "fp64_roofline_frac": nc * flops / (ms64 / 1e3) / 1e12 / PEAK_TFLOPS["fp64"],
+        }
+        if args.only == "sweep":
+            line = {"metric": "adjoint grads/sec (adaptive candidate sweep, FP32-bulk)", "value": sweep["fp32_grads_per_s"],
+                    "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms32,
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                    "data": "synthetic", "config": {"workload": sweep["workload"]}, "sweep": sweep}
+        else:
+            line["sweep_smf28_8192"] = sweep
     print(json.dumps(line), flush=True)
 
 

@@ -108,8 +108,8 @@ class Plan:
             D = D[:, None]
         self.nt, self.modes = D.shape
         self.precision = precision
-        if scheme is None:
-            scheme = "rk4ip" if steps else "tsit5"
+        if scheme is None:  # a step count selects the fixed-step scheme, none the adaptive one
+            scheme = "rk4ip" if steps is not None else "tsit5"
         if scheme not in SCHEMES:
             raise ValueError(f"scheme must be one of {sorted(SCHEMES)}")
         self.scheme = scheme
