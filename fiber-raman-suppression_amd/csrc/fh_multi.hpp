@@ -41,6 +41,8 @@ template <typename R> struct MmArgs {
     double h, length;
     int steps;
     int q;  // number of output pairs (M(M+1)/2 or M^2)
+    const double* lengths;  // [batch] per-solve fiber length, or null for `length`
+    const int* nsteps;      // [batch] per-solve step count, or null for `steps`
 };
 
 template <typename R, int N, int T, int M> struct MmTraits {
@@ -89,10 +91,12 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
             st.st(x, vo, CV::soff(k));
         }
     }
-    const double h = a.h;
+    const double Lb = a.lengths ? a.lengths[sol] : a.length;
+    const int Sb = a.nsteps ? a.nsteps[sol] : a.steps;
+    const double h = a.lengths ? Lb / Sb : a.h;
     const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
     V v[P];
-    for (int step = 0; step < a.steps; ++step) {
+    for (int step = 0; step < Sb; ++step) {
         const double z = step * h;
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R c, sn;
-            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), a.length, c, sn);
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, c, sn);
             const V x = st.template ld<V>(vo, CV::soff(k));
             const V o = V{x.x * c - x.y * sn, x.x * sn + x.y * c};
             bad |= !(isfinite(o.x) && isfinite(o.y));

@@ -1,0 +1,527 @@
+// fh_server.hip -- request server of libfiberhip: continuous batching of
+// independent forward / forward+adjoint solves (see fh_server_* in
+// include/fiberhip.h).
+//
+// One scheduler thread owns every HIP call of the server.  Requests that
+// share their operators form a batch (one fused launch, one workgroup per
+// solve, each solve with its own length); up to `lanes` batches run at once
+// on their own default-priority streams, each with an exclusive plan
+// instance (plans hold the per-solve scratch).  Plans are created on demand,
+// kept while idle for reuse, and released least-recently-used first when a
+// new plan would exceed the memory budget.  Inputs are staged through pinned
+// host buffers; outputs come back asynchronously on the lane's stream and are
+// scattered to the callers' buffers when the lane's completion event fires.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "fh_device.hpp"
+#include "fh_internal.hpp"
+#include "fh_launch.hpp"
+
+namespace {
+
+int64_t now_ns()
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n)
+    {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+    template <typename T> void val(T v) { add(&v, sizeof(v)); }
+};
+
+// Server-owned copy of one operator set (the batching key).
+struct OpSet {
+    int nt = 0, modes = 0;
+    bool grad = false;
+    std::vector<double> disp, gamma, raman, shock;
+    std::vector<uint8_t> mask;
+    double one_m_fr = 0;
+};
+
+struct Req {
+    int kind = 0;
+    uint64_t key = 0;
+    double length = 0;
+    int nsteps = 0;
+    std::vector<double> u0;
+    double *uL = nullptr, *grad = nullptr, *cost = nullptr;
+    int state = 0;  // 0 queued, 1 running, 2 finished
+    fh_request_result res{};
+};
+
+struct PlanInst {
+    fh_plan plan = nullptr;
+    uint64_t key = 0;
+    bool busy = false;
+    int64_t last_used = 0, bytes = 0;
+};
+
+template <typename T> struct Pinned {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want)
+    {
+        if (want <= n) return FH_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipHostMalloc((void**)&p, sizeof(T) * want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fhi::set_error(FH_ENOMEM, "hipHostMalloc of a staging buffer failed");
+        }
+        n = want;
+        return FH_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Lane {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    bool grad = false;
+    int modes = 1, nt = 0;
+    PlanInst* pi = nullptr;
+    std::vector<int64_t> ids;
+    Pinned<char> h_in, h_out;
+    Pinned<double> h_grad, h_cost, h_len;
+    Pinned<int> h_status, h_nst;
+    Pinned<long long> h_stats;
+    double* d_len = nullptr;
+    int* d_nst = nullptr;
+};
+
+}  // namespace
+
+struct fh_server_s {
+    fh_context ctx = nullptr;
+    fh_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::unordered_map<int64_t, Req> reqs;
+    std::unordered_map<uint64_t, OpSet> ops;
+    std::list<PlanInst> plans;
+    std::vector<Lane> lanes;
+    int64_t next_id = 1, plan_bytes = 0;
+    bool stop = false;
+    std::thread worker;
+};
+
+namespace {
+
+void finish(fh_server_s* S, int64_t id, int code)
+{
+    Req& r = S->reqs[id];
+    r.res.retcode = code;
+    r.res.finish_ns = now_ns();
+    r.state = 2;
+}
+
+// Release idle plans, least recently used first, until `need` more bytes fit.
+void evict_for(fh_server_s* S, int64_t need)
+{
+    while (S->plan_bytes + need > S->cfg.memory_budget_bytes) {
+        auto victim = S->plans.end();
+        for (auto it = S->plans.begin(); it != S->plans.end(); ++it)
+            if (!it->busy && (victim == S->plans.end() || it->last_used < victim->last_used)) victim = it;
+        if (victim == S->plans.end()) return;
+        S->plan_bytes -= victim->bytes;
+        fh_plan_destroy(victim->plan);
+        S->plans.erase(victim);
+    }
+}
+
+// An idle plan instance for `key`, created if needed.  Returns nullptr with
+// *wait = true when the budget is taken by running launches.
+PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
+{
+    *rc = FH_OK;
+    *wait = false;
+    for (auto& pi : S->plans)
+        if (pi.key == key && !pi.busy) return &pi;
+    const OpSet& o = S->ops[key];
+    fh_problem_desc d{};
+    d.nt = o.nt;
+    d.modes = o.modes;
+    d.precision = S->cfg.precision;
+    d.scheme = o.modes == 1 ? FH_SCHEME_TSIT5 : FH_SCHEME_RK4IP_FIXED;
+    d.steps = 1;  // multimode: per-solve step counts override
+    d.max_batch = S->cfg.max_batch;
+    d.length_m = 1e-3;  // per-solve lengths override; the phase bound is checked per request
+    d.dispersion = o.disp.data();
+    d.gamma = o.gamma.data();
+    d.raman = o.raman.data();
+    d.shock = o.shock.data();
+    d.one_m_fr = o.one_m_fr;
+    d.checkpoint_every = 50;
+    d.band_mask = o.grad ? o.mask.data() : nullptr;
+    d.rtol = S->cfg.rtol;
+    d.atol = S->cfg.atol;
+    d.max_steps = S->cfg.max_steps;
+    fh_plan p = nullptr;
+    *rc = fh_plan_create(S->ctx, &d, &p);
+    if (*rc == FH_OK && o.grad) *rc = fhi::plan_prepare_gradient(p);
+    if (*rc != FH_OK) {
+        if (p) fh_plan_destroy(p);
+        if (*rc == FH_ENOMEM) {  // fragmentation or a full device: retry after running work drains
+            for (auto& l : S->lanes)
+                if (l.busy) {
+                    *wait = true;
+                    *rc = FH_OK;
+                }
+        }
+        return nullptr;
+    }
+    const int64_t bytes = fhi::plan_bytes(p);
+    evict_for(S, bytes);
+    if (S->plan_bytes + bytes > S->cfg.memory_budget_bytes) {
+        fh_plan_destroy(p);
+        for (auto& l : S->lanes)
+            if (l.busy) {
+                *wait = true;
+                return nullptr;
+            }
+        *rc = fhi::set_error(FH_ENOMEM, "one launch of this request class needs " + std::to_string(bytes) +
+                                            " device bytes, above the server's memory budget");
+        return nullptr;
+    }
+    S->plans.push_back(PlanInst{p, key, false, now_ns(), bytes});
+    S->plan_bytes += bytes;
+    return &S->plans.back();
+}
+
+// Start one batch on an idle lane.  Returns true if work was dispatched or
+// failed requests were finished, false when it has to wait.
+bool dispatch(fh_server_s* S, Lane& L)
+{
+    const uint64_t key = S->reqs[S->queue.front()].key;
+    int rc;
+    bool wait;
+    PlanInst* pi = acquire_plan(S, key, &rc, &wait);
+    if (wait) return false;
+    std::vector<int64_t> ids;
+    for (auto it = S->queue.begin(); it != S->queue.end() && (int)ids.size() < S->cfg.max_batch;) {
+        if (S->reqs[*it].key == key) {
+            ids.push_back(*it);
+            it = S->queue.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    if (!pi) {
+        for (int64_t id : ids) finish(S, id, rc);
+        S->cv_done.notify_all();
+        return true;
+    }
+    const OpSet& o = S->ops[key];
+    const int B = (int)ids.size(), nf = o.nt * o.modes;
+    const bool fp64 = S->cfg.precision == FH_PREC_FP64;
+    const size_t es = fp64 ? sizeof(double) : sizeof(float);
+    int e = FH_OK;
+    if (!e) e = L.h_in.ensure((size_t)B * nf * 2 * es);
+    if (!e) e = L.h_out.ensure((size_t)B * nf * 2 * es);
+    if (!e) e = L.h_grad.ensure((size_t)B * o.nt);
+    if (!e) e = L.h_cost.ensure(B);
+    if (!e) e = L.h_len.ensure(B);
+    if (!e) e = L.h_status.ensure(B);
+    if (!e) e = L.h_nst.ensure(B);
+    if (!e) e = L.h_stats.ensure(3 * (size_t)B);
+    const int64_t t0 = now_ns();
+    for (int b = 0; e == FH_OK && b < B; ++b) {
+        Req& r = S->reqs[ids[b]];
+        if (fp64) {
+            std::memcpy(L.h_in.p + (size_t)b * nf * 2 * es, r.u0.data(), (size_t)nf * 2 * es);
+        } else {
+            float* dst = (float*)(L.h_in.p + (size_t)b * nf * 2 * es);
+            for (int i = 0; i < 2 * nf; ++i) dst[i] = (float)r.u0[i];
+        }
+        L.h_len.p[b] = r.length;
+        L.h_nst.p[b] = r.nsteps;
+        r.state = 1;
+        r.res.start_ns = t0;
+        r.res.batch = B;
+    }
+    if (e == FH_OK && hipMemcpyAsync(fhi::plan_u_in(pi->plan), L.h_in.p, (size_t)B * nf * 2 * es,
+                                     hipMemcpyHostToDevice, L.s) != hipSuccess)
+        e = fhi::set_error(FH_EDEVICE, "staging copy failed");
+    if (e == FH_OK && (hipMemcpyAsync(L.d_len, L.h_len.p, sizeof(double) * B, hipMemcpyHostToDevice, L.s) != hipSuccess ||
+                       hipMemcpyAsync(L.d_nst, L.h_nst.p, sizeof(int) * B, hipMemcpyHostToDevice, L.s) != hipSuccess))
+        e = fhi::set_error(FH_EDEVICE, "staging copy failed");
+    if (e == FH_OK)
+        e = fhi::enqueue_batch(pi->plan, B, o.grad, L.d_len, o.modes > 1 ? L.d_nst : nullptr, L.s, L.h_out.p,
+                               L.h_grad.p, L.h_cost.p, L.h_status.p, L.h_stats.p);
+    if (e == FH_OK && hipEventRecord(L.ev, L.s) != hipSuccess) e = fhi::set_error(FH_EDEVICE, "hipEventRecord failed");
+    if (e != FH_OK) {
+        (void)hipStreamSynchronize(L.s);
+        for (int64_t id : ids) finish(S, id, e);
+        pi->last_used = now_ns();
+        S->cv_done.notify_all();
+        return true;
+    }
+    pi->busy = true;
+    L.busy = true;
+    L.pi = pi;
+    L.ids = std::move(ids);
+    L.grad = o.grad;
+    L.modes = o.modes;
+    L.nt = o.nt;
+    return true;
+}
+
+void complete(fh_server_s* S, Lane& L, int device_rc)
+{
+    const bool fp64 = S->cfg.precision == FH_PREC_FP64;
+    const size_t es = fp64 ? sizeof(double) : sizeof(float);
+    const int nf = L.nt * L.modes;
+    const bool adaptive = fhi::plan_adaptive(L.pi->plan);
+    for (size_t b = 0; b < L.ids.size(); ++b) {
+        Req& r = S->reqs[L.ids[b]];
+        int code = device_rc;
+        if (code == FH_OK) {
+            const int st = L.h_status.p[b];
+            code = (st & 2) ? FH_EMAXSTEPS : (st ? FH_ENONFINITE : FH_OK);
+            if (fp64) {
+                std::memcpy(r.uL, L.h_out.p + b * nf * 2 * es, (size_t)nf * 2 * es);
+            } else {
+                const float* src = (const float*)(L.h_out.p + b * nf * 2 * es);
+                for (int i = 0; i < 2 * nf; ++i) r.uL[i] = src[i];
+            }
+            if (L.grad) {
+                if (r.grad) std::memcpy(r.grad, L.h_grad.p + b * L.nt, sizeof(double) * L.nt);
+                if (r.cost) r.cost[0] = L.h_cost.p[b];
+            }
+            if (adaptive) {
+                r.res.accepted = L.h_stats.p[3 * b];
+                r.res.rejected = L.h_stats.p[3 * b + 1];
+                r.res.rhs_evals = L.h_stats.p[3 * b + 2];
+            } else {
+                r.res.accepted = r.nsteps;
+                r.res.rhs_evals = 4LL * r.nsteps;
+            }
+        }
+        finish(S, L.ids[b], code);
+    }
+    L.pi->busy = false;
+    L.pi->last_used = now_ns();
+    L.pi = nullptr;
+    L.ids.clear();
+    L.busy = false;
+    S->cv_done.notify_all();
+}
+
+void worker_main(fh_server_s* S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        bool progressed = false;
+        for (auto& L : S->lanes) {
+            if (!L.busy) continue;
+            const hipError_t q = hipEventQuery(L.ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) (void)hipStreamSynchronize(L.s);
+            complete(S, L, q == hipSuccess ? FH_OK : fhi::set_error(FH_EDEVICE, hipGetErrorString(q)));
+            progressed = true;
+        }
+        for (auto& L : S->lanes) {
+            if (L.busy || S->queue.empty()) continue;
+            if (!dispatch(S, L)) break;
+            progressed = true;
+        }
+        bool any_busy = false;
+        for (auto& L : S->lanes) any_busy |= L.busy;
+        if (S->stop && S->queue.empty() && !any_busy) break;
+        if (!progressed) S->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? 50 : 2000));
+    }
+}
+
+int validate(const fh_server_s* S, const fh_request* r, double* dmax)
+{
+    if (!r) return fhi::set_error(FH_EINVAL, "request is NULL");
+    if (r->kind != FH_REQ_FORWARD && r->kind != FH_REQ_GRADIENT) return fhi::set_error(FH_EINVAL, "unknown request kind");
+    if (r->nt < 4 || (r->nt & (r->nt - 1)) || r->modes < 1) return fhi::set_error(FH_EINVAL, "nt must be a power of two, modes >= 1");
+    if (!r->dispersion || !r->gamma || !r->raman || !r->shock || !r->u0 || !r->uL)
+        return fhi::set_error(FH_EINVAL, "operator, launch or output pointer is NULL");
+    if (!(std::isfinite(r->length_m) && r->length_m > 0)) return fhi::set_error(FH_EINVAL, "fiber length must be positive and finite");
+    if (r->kind == FH_REQ_GRADIENT) {
+        if (r->modes > 1) return fhi::set_error(FH_EUNSUPPORTED, "multimode gradients are not offered by the server");
+        if (!r->band_mask) return fhi::set_error(FH_EINVAL, "gradient requests need a band mask");
+    }
+    const bool fp64 = S->cfg.precision == FH_PREC_FP64;
+    if (r->modes == 1 ? !fh::tsit5_supported(r->nt, 1) : !fh::multi_supported(r->nt, r->modes, fp64))
+        return fhi::set_error(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(r->nt) +
+                                                   " with " + std::to_string(r->modes) + " modes");
+    const size_t nf = (size_t)r->nt * r->modes;
+    double m = 0;
+    for (size_t i = 0; i < nf; ++i) {
+        if (!std::isfinite(r->dispersion[i])) return fhi::set_error(FH_EINVAL, "dispersion must be finite");
+        m = std::fmax(m, std::fabs(r->dispersion[i]));
+    }
+    if (m * r->length_m >= FH_MAX_PHASE) return fhi::set_error(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds the bound");
+    for (size_t i = 0; i < 2 * nf; ++i)
+        if (!std::isfinite(r->u0[i])) return fhi::set_error(FH_EINVAL, "input spectrum must be finite");
+    *dmax = m;
+    return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out)
+{
+    if (!ctx || !cfg || !out) return fhi::set_error(FH_EINVAL, "context, config or out is NULL");
+    if (cfg->lanes < 1 || cfg->lanes > 16) return fhi::set_error(FH_EINVAL, "lanes must lie in [1, 16]");
+    if (cfg->max_batch < 1) return fhi::set_error(FH_EINVAL, "max_batch must be positive");
+    if (cfg->memory_budget_bytes <= 0) return fhi::set_error(FH_EINVAL, "memory budget must be positive");
+    if (cfg->precision != FH_PREC_FP64 && cfg->precision != FH_PREC_FP32_BULK) return fhi::set_error(FH_EINVAL, "unknown precision");
+    if (!(cfg->rtol > 0) || !(cfg->atol >= 0) || !(cfg->multimode_step_m > 0))
+        return fhi::set_error(FH_EINVAL, "tolerances and the multimode step must be positive");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fhi::set_error(FH_EDEVICE, "hipSetDevice failed");
+    fh_server_s* S = new fh_server_s;
+    S->ctx = ctx;
+    S->cfg = *cfg;
+    S->lanes.resize(cfg->lanes);
+    for (auto& L : S->lanes) {
+        // non-blocking: plan uploads on the null stream do not wait for running lanes
+        bool ok = hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking) == hipSuccess && hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc((void**)&L.d_len, sizeof(double) * cfg->max_batch) == hipSuccess &&
+                  hipMalloc((void**)&L.d_nst, sizeof(int) * cfg->max_batch) == hipSuccess;
+        if (!ok) {
+            fh_server_destroy(S);
+            return fhi::set_error(FH_EDEVICE, "lane setup failed");
+        }
+    }
+    S->worker = std::thread(worker_main, S);
+    *out = S;
+    return FH_OK;
+}
+
+int fh_server_submit(fh_server S, const fh_request* r, int64_t* id)
+{
+    if (!S || !id) return fhi::set_error(FH_EINVAL, "server or id is NULL");
+    double dmax = 0;
+    int rc = validate(S, r, &dmax);
+    if (rc) return rc;
+    const bool grad = r->kind == FH_REQ_GRADIENT;
+    const size_t nf = (size_t)r->nt * r->modes, m4 = (size_t)r->modes * r->modes * r->modes * r->modes;
+    Fnv h;
+    h.val(grad);
+    h.val(r->nt);
+    h.val(r->modes);
+    h.val(r->one_m_fr);
+    h.add(r->dispersion, sizeof(double) * nf);
+    h.add(r->gamma, sizeof(double) * m4);
+    h.add(r->raman, sizeof(double) * 2 * r->nt);
+    h.add(r->shock, sizeof(double) * r->nt);
+    if (grad) h.add(r->band_mask, (size_t)r->nt);
+    Req q;
+    q.kind = r->kind;
+    q.key = h.h;
+    q.length = r->length_m;
+    q.nsteps = r->modes > 1 ? std::max(16, (int)std::ceil(r->length_m / S->cfg.multimode_step_m)) : 0;
+    q.u0.assign(r->u0, r->u0 + 2 * nf);
+    q.uL = r->uL;
+    q.grad = r->grad;
+    q.cost = r->cost;
+    q.res.submit_ns = now_ns();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return fhi::set_error(FH_EINVAL, "server is shutting down");
+    if (!S->ops.count(q.key)) {
+        OpSet o;
+        o.nt = r->nt;
+        o.modes = r->modes;
+        o.grad = grad;
+        o.disp.assign(r->dispersion, r->dispersion + nf);
+        o.gamma.assign(r->gamma, r->gamma + m4);
+        o.raman.assign(r->raman, r->raman + 2 * r->nt);
+        o.shock.assign(r->shock, r->shock + r->nt);
+        if (grad) o.mask.assign(r->band_mask, r->band_mask + r->nt);
+        o.one_m_fr = r->one_m_fr;
+        S->ops.emplace(q.key, std::move(o));
+    }
+    *id = S->next_id++;
+    S->reqs.emplace(*id, std::move(q));
+    S->queue.push_back(*id);
+    S->cv_work.notify_one();
+    return FH_OK;
+}
+
+int fh_server_poll(fh_server S, int64_t id, int32_t* state)
+{
+    if (!S || !state) return fhi::set_error(FH_EINVAL, "server or state is NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fhi::set_error(FH_EINVAL, "unknown request id");
+    *state = it->second.state;
+    return FH_OK;
+}
+
+int fh_server_wait(fh_server S, int64_t id, fh_request_result* res)
+{
+    if (!S || !res) return fhi::set_error(FH_EINVAL, "server or result is NULL");
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (!S->reqs.count(id)) return fhi::set_error(FH_EINVAL, "unknown request id");
+    S->cv_done.wait(lk, [&] { return S->reqs[id].state == 2; });
+    *res = S->reqs[id].res;
+    S->reqs.erase(id);
+    if (res->retcode) return fhi::set_error(res->retcode, "request " + std::to_string(id) + " failed");
+    return FH_OK;
+}
+
+int fh_server_destroy(fh_server S)
+{
+    if (!S) return FH_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->ctx->device);
+    for (auto& pi : S->plans) fh_plan_destroy(pi.plan);
+    for (auto& L : S->lanes) {
+        if (L.s) (void)hipStreamSynchronize(L.s);
+        L.h_in.release();
+        L.h_out.release();
+        L.h_grad.release();
+        L.h_cost.release();
+        L.h_len.release();
+        L.h_status.release();
+        L.h_nst.release();
+        L.h_stats.release();
+        if (L.d_len) (void)hipFree(L.d_len);
+        if (L.d_nst) (void)hipFree(L.d_nst);
+        if (L.ev) (void)hipEventDestroy(L.ev);
+        if (L.s) (void)hipStreamDestroy(L.s);
+    }
+    delete S;
+    return FH_OK;
+}
+
+}  // extern "C"

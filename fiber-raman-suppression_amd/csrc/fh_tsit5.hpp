@@ -47,6 +47,8 @@ template <typename R> struct TsArgs {
     double* hs;        // [batch][max_steps] accepted step sizes
     double* zs;        // [batch][max_steps] position before each accepted step
     long long* stats;  // [batch][3] accepted, rejected, rhs evaluations
+    const double* lengths;  // [batch] fiber length per solve, or null for s.length
+    __device__ __forceinline__ double length(size_t sol) const { return lengths ? lengths[sol] : s.length; }
 };
 
 // Slot map of the workgroup slice.  Forward: y, ynew, k0..k6.  Gradient adds
@@ -91,7 +93,7 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     const Buf none(a.s.u0, 0);
     const Fld nf{none, 0};
     const size_t sol = blockIdx.x;
-    const double L = a.s.length, rtol = a.rtol, atol = a.atol;
+    const double L = a.length(blockIdx.x), rtol = a.rtol, atol = a.atol;
     int sy = LY::Y, syn = LY::YN;
     // k0 and k6 trade slots on acceptance (first same as last); k1..k5 are fixed
     int sk0 = LY::K0, sk6 = LY::K0 + 6;
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(T) void k_tsit5_single(TsArgs<R> a)
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         R c, s;
-        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.s.length, c, s);
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length(sol), c, s);
         const V x = y.template ld<V>(vo, CV::soff(k));
         const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
         if (!(isfinite(o.x) && isfinite(o.y))) bad |= 1;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R c, s;
-            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), sa.length, c, s);
+            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length(sol), c, s);
             const V x = y.template ld<V>(vo, CV::soff(k));
             const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
             if (!(isfinite(o.x) && isfinite(o.y))) bad |= 1;
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R c, s;
-            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), sa.length, c, s);
+            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length(sol), c, s);
             lam.st(V{c * v[k].x + s * v[k].y, c * v[k].y - s * v[k].x}, vo, CV::soff(k));
         }
     }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/fiberhip.h"
+#include "fh_internal.hpp"
 #include "fh_launch.hpp"
 #include "fh_multi.hpp"
 #include "fh_single.hpp"
@@ -34,11 +35,6 @@ int fail(int code, const std::string& msg)
     } while (0)
 
 }  // namespace
-
-struct fh_context_s {
-    int device;
-    hipStream_t stream;
-};
 
 struct DevBuf {
     void* p = nullptr;
@@ -405,7 +401,7 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
 
 template <typename R>
 int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
-                    double* d_grad, double* d_cost, hipStream_t s)
+                    double* d_grad, double* d_cost, hipStream_t s, const double* d_lengths = nullptr)
 {
     if (p->d.modes != 1) return fail(FH_EUNSUPPORTED, "the adjoint kernel of this build is single-mode");
     if (!d_seed && !p->has_mask)
@@ -423,13 +419,18 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
     a.tape_u = (fh::C2<R>*)p->tape_u.p;
     a.tape_h = (R*)p->tape_h.p;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
-    hipError_t e = p->adaptive ? fh::launch_tsit5(p->d.nt, true, ts_args<R>(p, a), batch, s)
+    if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
+    fh::TsArgs<R> ta = ts_args<R>(p, a);
+    ta.lengths = d_lengths;
+    hipError_t e = p->adaptive ? fh::launch_tsit5(p->d.nt, true, ta, batch, s)
                                : with_size<R, OpGrad>(p->d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("gradient launch: ") + hipGetErrorString(e));
     return FH_OK;
 }
 
-template <typename R> int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+template <typename R>
+int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s,
+                         const double* d_lengths = nullptr, const int* d_nsteps = nullptr)
 {
     const fh_problem_desc& d = p->d;
     fh::MmArgs<R> a{};
@@ -448,21 +449,28 @@ template <typename R> int launch_forward_multi(fh_plan p, int batch, const void*
     a.length = d.length_m;
     a.steps = d.steps;
     a.q = p->q;
+    a.lengths = d_lengths;
+    a.nsteps = d_nsteps;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
     hipError_t e = fh::launch_multi(d.nt, d.modes, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode launch: ") + hipGetErrorString(e));
     return FH_OK;
 }
 
-template <typename R> int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s)
+template <typename R>
+int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s,
+                   const double* d_lengths = nullptr, const int* d_nsteps = nullptr)
 {
     const fh_problem_desc& d = p->d;
-    if (d.modes > 1) return launch_forward_multi<R>(p, batch, d_u0, d_uL, s);
+    if (d.modes > 1) return launch_forward_multi<R>(p, batch, d_u0, d_uL, s, d_lengths, d_nsteps);
+    if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
     fh::SmArgs<R> a = base_args<R>(p);
     a.u0 = (const fh::C2<R>*)d_u0;
     a.out = (fh::C2<R>*)d_uL;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
-    hipError_t e = p->adaptive ? fh::launch_tsit5(d.nt, false, ts_args<R>(p, a), batch, s)
+    fh::TsArgs<R> ta = ts_args<R>(p, a);
+    ta.lengths = d_lengths;
+    hipError_t e = p->adaptive ? fh::launch_tsit5(d.nt, false, ta, batch, s)
                                : with_size<R, OpLaunch>(d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     return FH_OK;
@@ -511,6 +519,61 @@ int finish_call(fh_plan p, int batch, const std::vector<int>& status, fh_stats* 
 }
 
 }  // namespace
+
+// Internal interface for the request server (fh_server.hip); C++ linkage.
+namespace fhi {
+
+bool plan_fp64(fh_plan p) { return p->d.precision == FH_PREC_FP64; }
+int plan_nt(fh_plan p) { return p->d.nt; }
+int plan_modes(fh_plan p) { return p->d.modes; }
+int plan_max_batch(fh_plan p) { return p->d.max_batch; }
+bool plan_adaptive(fh_plan p) { return p->adaptive; }
+void* plan_u_in(fh_plan p) { return p->u_in.p; }
+void* plan_u_out(fh_plan p) { return p->u_out.p; }
+int64_t plan_bytes(fh_plan p) { return p->total_bytes; }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+// Allocate the adjoint buffers now, so the plan's footprint is final.
+int plan_prepare_gradient(fh_plan p)
+{
+    FH_HIP(hipSetDevice(p->ctx->device));
+    return plan_fp64(p) ? ensure_adjoint_buffers<double>(p) : ensure_adjoint_buffers<float>(p);
+}
+
+// Enqueue one batch on `s` from the plan's own input buffer (u_in) with
+// per-solve lengths / step counts (device arrays, may be null), followed by
+// asynchronous copies of the outputs and counters into host (pinned) memory:
+// uL [batch][modes][nt] in the plan's precision, grad [batch][nt] and cost
+// [batch] (gradient), status [batch] and stats [batch][3].
+int enqueue_batch(fh_plan p, int batch, bool grad, const double* d_lengths, const int* d_nsteps, hipStream_t s,
+                  void* h_uL, double* h_grad, double* h_cost, int* h_status, long long* h_stats)
+{
+    const bool fp64 = plan_fp64(p);
+    int rc;
+    if (grad) {
+        if ((rc = plan_prepare_gradient(p))) return rc;
+        rc = fp64 ? launch_gradient<double>(p, batch, p->u_in.p, nullptr, p->u_out.p, p->g_lam0.p,
+                                            (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths)
+                  : launch_gradient<float>(p, batch, p->u_in.p, nullptr, p->u_out.p, p->g_lam0.p,
+                                           (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths);
+    } else {
+        rc = fp64 ? launch_forward<double>(p, batch, p->u_in.p, p->u_out.p, s, d_lengths, d_nsteps)
+                  : launch_forward<float>(p, batch, p->u_in.p, p->u_out.p, s, d_lengths, d_nsteps);
+    }
+    if (rc) return rc;
+    const size_t n = (size_t)batch * p->d.nt * p->d.modes;
+    FH_HIP(hipMemcpyAsync(h_uL, p->u_out.p, n * 2 * (fp64 ? sizeof(double) : sizeof(float)), hipMemcpyDeviceToHost, s));
+    if (grad) {
+        FH_HIP(hipMemcpyAsync(h_grad, p->g_grad.p, sizeof(double) * (size_t)batch * p->d.nt, hipMemcpyDeviceToHost, s));
+        FH_HIP(hipMemcpyAsync(h_cost, p->g_cost.p, sizeof(double) * (size_t)batch, hipMemcpyDeviceToHost, s));
+    }
+    FH_HIP(hipMemcpyAsync(h_status, p->status.p, sizeof(int) * (size_t)batch, hipMemcpyDeviceToHost, s));
+    if (p->adaptive)
+        FH_HIP(hipMemcpyAsync(h_stats, p->sstats.p, sizeof(long long) * 3 * (size_t)batch, hipMemcpyDeviceToHost, s));
+    return FH_OK;
+}
+
+}  // namespace fhi
 
 extern "C" {
 

@@ -8,6 +8,7 @@ the host the way the reference's helpers do and moves fields across the ABI.
 """
 from ._lib import FiberHIPError, FiberHIPUnavailable, exported_names, lib  # noqa: F401
 from .engine import Context, DeviceArray, Plan, default_context  # noqa: F401
+from .serving import Server  # noqa: F401
 from .problem import (  # noqa: F401
     Fiber,
     FiberFieldProblem,

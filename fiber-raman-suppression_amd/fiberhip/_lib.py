@@ -82,6 +82,55 @@ class Stats(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("memory_budget_bytes", ctypes.c_int64),
+        ("lanes", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("max_steps", ctypes.c_int32),
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("multimode_step_m", ctypes.c_double),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("modes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("length_m", ctypes.c_double),
+        ("dispersion", ctypes.POINTER(ctypes.c_double)),
+        ("gamma", ctypes.POINTER(ctypes.c_double)),
+        ("raman", ctypes.POINTER(ctypes.c_double)),
+        ("shock", ctypes.POINTER(ctypes.c_double)),
+        ("one_m_fr", ctypes.c_double),
+        ("band_mask", ctypes.POINTER(ctypes.c_uint8)),
+        ("u0", ctypes.POINTER(ctypes.c_double)),
+        ("uL", ctypes.POINTER(ctypes.c_double)),
+        ("grad", ctypes.POINTER(ctypes.c_double)),
+        ("cost", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("submit_ns", ctypes.c_int64),
+        ("start_ns", ctypes.c_int64),
+        ("finish_ns", ctypes.c_int64),
+        ("accepted", ctypes.c_int64),
+        ("rejected", ctypes.c_int64),
+        ("rhs_evals", ctypes.c_int64),
+        ("retcode", ctypes.c_int32),
+        ("batch", ctypes.c_int32),
+    ]
+
+
+FH_REQ_FORWARD = 0
+FH_REQ_GRADIENT = 1
+
 _LIB = None
 
 _VP = ctypes.c_void_p
@@ -109,6 +158,11 @@ _SIGNATURES = {
                                ctypes.c_int),
     "fh_solve_stats": ([_VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "fh_step_sizes": ([_VP, ctypes.c_int32, _DP], ctypes.c_int),
+    "fh_server_create": ([_VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)], ctypes.c_int),
+    "fh_server_submit": ([_VP, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_server_poll": ([_VP, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "fh_server_wait": ([_VP, ctypes.c_int64, ctypes.POINTER(RequestResult)], ctypes.c_int),
+    "fh_server_destroy": ([_VP], ctypes.c_int),
     "fh_last_error": ([], ctypes.c_char_p),
 }
 

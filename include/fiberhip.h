@@ -191,6 +191,72 @@ int fh_solve_stats(fh_plan plan, int32_t batch, int64_t *stats);
  * hs[b*max_steps + i], i < accepted steps of solve b (FH_SCHEME_TSIT5). */
 int fh_step_sizes(fh_plan plan, int32_t batch, double *hs);
 
+/* ------------------------------------------------------------ serving
+ *
+ * A persistent request server on one context: independent forward or
+ * forward+adjoint solves of uneven nt, mode count and fiber length are
+ * submitted from any thread; a scheduler thread batches requests that share
+ * their operators (same dispersion, gamma, Raman response, shock and band
+ * mask) into one fused launch, runs up to `lanes` launches concurrently on
+ * default-priority streams, and admits a launch only while the device memory
+ * of the plans it needs stays within `memory_budget_bytes` (idle plans are
+ * released least-recently-used first).  Each solve's result depends only on
+ * its own inputs, so it is the same alone and inside any batch.
+ * Single-mode requests run the adaptive Tsit5 scheme (config rtol/atol) with
+ * their own length; multimode requests run the fixed-step RK4IP with
+ * ceil(length / multimode_step_m) steps (at least 16); multimode gradients are
+ * not offered (FH_EUNSUPPORTED).  No counterpart exists in the reference (its
+ * solves run one at a time in the calling Julia task); callers are batch
+ * optimisation drivers and services that issue many solves. */
+
+typedef struct fh_server_s *fh_server;
+
+typedef struct {
+    int64_t memory_budget_bytes; /* device memory the server's plans may hold      */
+    int32_t lanes;               /* concurrent launches (streams), >= 1            */
+    int32_t max_batch;           /* solves per launch                              */
+    int32_t precision;           /* FH_PREC_* of every request                     */
+    int32_t max_steps;           /* adaptive accepted-step limit (0 = 10000)       */
+    double rtol, atol;           /* adaptive tolerances of single-mode requests    */
+    double multimode_step_m;     /* fixed step of multimode requests               */
+} fh_server_config;
+
+enum { FH_REQ_FORWARD = 0, FH_REQ_GRADIENT = 1 };
+
+/* One request.  Operators as in fh_problem_desc (host, copied at submit);
+ * u0 (nt*modes complex128) is copied at submit.  Outputs are written at
+ * completion and must stay valid until fh_server_wait returns: uL
+ * (nt*modes complex128, lab frame), and for FH_REQ_GRADIENT the band
+ * objective's grad (nt, FP64 phase gradient) and cost (1). */
+typedef struct {
+    int32_t kind;  /* FH_REQ_*                  */
+    int32_t nt, modes;
+    int32_t reserved;
+    double length_m;
+    const double *dispersion, *gamma, *raman, *shock;
+    double one_m_fr;
+    const uint8_t *band_mask; /* FH_REQ_GRADIENT: nt bins */
+    const double *u0;
+    double *uL, *grad, *cost;
+} fh_request;
+
+typedef struct {
+    int64_t submit_ns, start_ns, finish_ns; /* steady-clock times of submission, launch, completion */
+    int64_t accepted, rejected, rhs_evals;  /* this solve's counters                                */
+    int32_t retcode;                        /* FH_OK or the solve's error code                      */
+    int32_t batch;                          /* solves in the launch that carried it                 */
+} fh_request_result;
+
+int fh_server_create(fh_context ctx, const fh_server_config *cfg, fh_server *out);
+/* Validates and queues one request; returns its id.  Thread-safe. */
+int fh_server_submit(fh_server srv, const fh_request *req, int64_t *id);
+/* state: 0 queued, 1 running, 2 finished.  Thread-safe. */
+int fh_server_poll(fh_server srv, int64_t id, int32_t *state);
+/* Blocks until the request finished, fills `res` and forgets the id. */
+int fh_server_wait(fh_server srv, int64_t id, fh_request_result *res);
+/* Finishes every queued request, then stops the scheduler and frees the plans. */
+int fh_server_destroy(fh_server srv);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *fh_last_error(void);
 
