@@ -174,6 +174,90 @@ def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol
     return dict(ms=float(np.mean(sweeps)), stats=np.concatenate(stats), cost=np.concatenate(costs))
 
 
+SERVE_WINDOW_PS = {2048: 5.883, 4096: 11.766, 8192: 23.531, 16384: 47.061, 32768: 94.122}
+
+
+def serving_trace(n: int, seed: int = 5):
+    """Request mix: Nt uniform over 2^11..2^15 (carrier-margin windows), M uniform
+    over {1, 3, 6, 10} (SMF-28 for M = 1, 50 um GRIN with HG modes otherwise),
+    length log-uniform in [0.1, 5] m, peak power log-uniform in [1, 20] kW, launch
+    phase from the GDD/TOD/FOD distributions, half forward and half forward+adjoint."""
+    rng = np.random.default_rng(seed)
+    nts = rng.choice(list(SERVE_WINDOW_PS), n)
+    ms = rng.choice([1, 3, 6, 10], n)
+    lengths = np.exp(rng.uniform(np.log(0.1), np.log(5.0), n))
+    peaks = np.exp(rng.uniform(np.log(1e3), np.log(2e4), n))
+    coef = np.stack([rng.uniform(-1e4, 1e4, n), rng.uniform(-5e5, 5e5, n), rng.uniform(-1e7, 1e7, n)], axis=1)
+    kinds = np.where(rng.uniform(size=n) < 0.5, "forward", "gradient")
+    return [dict(nt=int(a), modes=int(b), length=float(c), peak=float(d), coef=e, kind=str(k))
+            for a, b, c, d, e, k in zip(nts, ms, lengths, peaks, coef, kinds)]
+
+
+def serving_problem(cache: dict, nt: int, modes: int):
+    import fiberhip
+
+    key = (nt, modes)
+    if key not in cache:
+        if modes == 1:
+            fib = fiberhip.Fiber(length_m=1.0, power_w=0.1, preset="SMF28", beta_order=3)
+            p = fiberhip.fiber_problem(fib, grid=fiberhip.Grid(nt=nt, time_window_ps=SERVE_WINDOW_PS[nt], policy="exact"),
+                                       raman_threshold_thz=None)
+        else:
+            from fiberhip import grin
+
+            p = grin.grin_problem(modes, nt=nt, time_window_ps=SERVE_WINDOW_PS[nt], length_m=1.0)
+        ut = np.fft.fft(p.uω0, axis=0)
+        cache[key] = (p, float(np.max(np.sum(np.abs(ut) ** 2, axis=1))), fiberhip.taylor_phase_basis(p, (2, 3, 4)),
+                      fiberhip.frequency_band_mask(p, (-16.0, -10.0)))
+    return cache[key]
+
+
+def run_serving(trace, precision, rate, cache, lanes=8, max_batch=32):
+    """Submit the trace with Poisson arrivals at `rate` requests/s from a feeder
+    thread and wait for every request; returns per-request results."""
+    import fiberhip
+
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(trace)))
+    inputs = []
+    for r in trace:
+        p, peak, basis, mask = serving_problem(cache, r["nt"], r["modes"])
+        u0 = p.uω0 * np.sqrt(r["peak"] / peak) * np.exp(1j * (basis @ r["coef"]))[:, None]
+        inputs.append((p, u0, mask))
+    ids, rejected = [None] * len(trace), {}
+    srv = fiberhip.Server(memory_budget_bytes=96 << 30, lanes=lanes, max_batch=max_batch, precision=precision,
+                          rtol=1e-6, atol=1e-6, multimode_step_m=5e-3, max_steps=20000)
+
+    def feed():
+        t0 = time.perf_counter()
+        for i, (r, (p, u0, mask)) in enumerate(zip(trace, inputs)):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            try:
+                ids[i] = srv.submit(p, kind=r["kind"], u0=u0, length_m=r["length"],
+                                    band_mask=mask if r["kind"] == "gradient" else None)
+            except fiberhip.FiberHIPError as exc:
+                cls = f"M={r['modes']} {r['kind']} nt={r['nt']}" if exc.code == fiberhip._lib.FH_EUNSUPPORTED else str(exc)
+                rejected[cls] = rejected.get(cls, 0) + 1
+
+    t_start = time.perf_counter()
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    results = [None] * len(trace)
+    for i, rid in enumerate(ids):
+        if rid is None:
+            continue
+        try:
+            results[i] = srv.wait(rid)
+        except fiberhip.FiberHIPError as exc:
+            results[i] = dict(error=str(exc))
+    wall = time.perf_counter() - t_start
+    srv.close()
+    return results, rejected, wall
+
+
 def cpu_baseline_gradient(problem, steps, mask, slice_steps=200):
     """The oracle's C discrete adjoint (FP64, one thread per gradient) on the
     host cores.  Bounded sample: every thread runs one gradient over the first
@@ -256,8 +340,10 @@ def main() -> None:
     ap.add_argument("--batch", type=int, default=256, help="gradients per launch (SMF-28 2 m gradient workload)")
     ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (SMF-28 0.5 m forward workload)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep"], default="all")
+    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "serving"], default="all")
     ap.add_argument("--sweep-candidates", type=int, default=256, help="adaptive candidate sweep size")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests in the serving trace")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
@@ -423,6 +509,46 @@ def main() -> None:
                     "data": "synthetic", "config": {"workload": sweep["workload"]}, "sweep": sweep}
         else:
             line["sweep_smf28_8192"] = sweep
+    if args.only in ("all", "serving"):
+        trace = serving_trace(args.serve_requests)
+        cache = {}
+        runs = {}
+        for precision in ("fp32", "fp64"):
+            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache)
+            ok = [r for r in res if r is not None and "error" not in r]
+            lat = np.array([r["latency_ms"] for r in ok])
+            runs[precision] = dict(results=res, rejected=rejected, wall_s=wall, served=len(ok),
+                                   failed=sum(1 for r in res if r is not None and "error" in r),
+                                   p50_ms=float(np.percentile(lat, 50)) if lat.size else None,
+                                   p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
+                                   solves_per_s=len(ok) / wall,
+                                   mean_batch=float(np.mean([r["batch"] for r in ok])) if ok else None)
+            print(f"bench: serving {precision}: {len(ok)} served in {wall:.1f} s, p50 {runs[precision]['p50_ms']} ms, "
+                  f"p99 {runs[precision]['p99_ms']} ms", file=sys.stderr, flush=True)
+        errs = []
+        for a, b in zip(runs["fp32"]["results"], runs["fp64"]["results"]):
+            if a and b and "error" not in a and "error" not in b:
+                ea, eb = np.sum(np.abs(a["uL"]) ** 2, axis=0), np.sum(np.abs(b["uL"]) ** 2, axis=0)
+                errs.append(float(np.max(np.abs(ea / eb - 1))))
+        serving = {
+            "workload": f"serving trace: {args.serve_requests} requests, Poisson arrivals at {args.serve_rate}/s, "
+                        f"Nt in 2^11..2^15, M in {{1,3,6,10}} (SMF-28 / 50 um GRIN HG), L log-uniform 0.1-5 m, "
+                        f"peak 1-20 kW, GDD/TOD/FOD phases, 50% forward / 50% forward+adjoint; single mode adaptive "
+                        f"Tsit5 rtol=atol=1e-6, multimode fixed 5 mm RK4IP steps",
+            **{f"{p}_{k}": runs[p][k] for p in ("fp32", "fp64")
+               for k in ("served", "failed", "p50_ms", "p99_ms", "solves_per_s", "wall_s", "mean_batch", "rejected")},
+            "fp32_vs_fp64_matched_requests": len(errs),
+            "fp32_vs_fp64_max_rel_mode_energy_err": max(errs) if errs else None,
+            "fp32_vs_fp64_p99_rel_mode_energy_err": float(np.percentile(errs, 99)) if errs else None,
+        }
+        if args.only == "serving":
+            line = {"metric": "serving solves/sec with p50/p99 latency (FP32-bulk)", "value": serving["fp32_solves_per_s"],
+                    "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+                    "ms_per_step": serving["fp32_wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": serving["workload"]},
+                    "serving": serving}
+        else:
+            line["serving_trace"] = serving
     print(json.dumps(line), flush=True)
 
 
