@@ -22,7 +22,10 @@ hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hi
 
 bool multi_supported(int nt, int m, bool fp64);
 int multi_fields(int m);
+int multi_grad_fields(int m);
 hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s);
+hipError_t launch_multi_grad(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_multi_grad(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s);
 
 }  // namespace fh

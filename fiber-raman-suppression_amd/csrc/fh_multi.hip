@@ -1,5 +1,6 @@
-// fh_multi.hip -- instantiations of the multimode kernel (kept in their own
-// translation unit so they compile in parallel with the single-mode ones).
+// fh_multi.hip -- instantiations of the multimode propagation and gradient
+// kernels (kept in their own translation unit so they compile in parallel
+// with the single-mode ones).
 #include <hip/hip_runtime.h>
 
 #include "fh_launch.hpp"
@@ -7,36 +8,43 @@
 
 namespace {
 
-template <typename R, int N, int M> hipError_t go(const fh::MmArgs<R>& a, int batch, hipStream_t s)
+template <typename R, int N, int M> hipError_t go(bool grad, const fh::MmArgs<R>& a, int batch, hipStream_t s)
 {
-    constexpr int T = fh::SingleShape<R, N>::T;
-    hipLaunchKernelGGL((fh::k_rk4ip_multi<R, N, T, M>), dim3(batch), dim3(T), 0, s, a);
-    return hipGetLastError();
+    if constexpr (sizeof(R) == 8 && M == 2) {
+        return hipErrorInvalidValue;  // not offered (multi_supported)
+    } else {
+        constexpr int T = fh::SingleShape<R, N>::T;
+        if (grad)
+            hipLaunchKernelGGL((fh::k_grad_multi<R, N, T, M>), dim3(batch), dim3(T), 0, s, a);
+        else
+            hipLaunchKernelGGL((fh::k_rk4ip_multi<R, N, T, M>), dim3(batch), dim3(T), 0, s, a);
+        return hipGetLastError();
+    }
 }
 
-template <typename R, int N> hipError_t by_modes(int m, const fh::MmArgs<R>& a, int batch, hipStream_t s)
+template <typename R, int N> hipError_t by_modes(bool grad, int m, const fh::MmArgs<R>& a, int batch, hipStream_t s)
 {
     switch (m) {
-    case 2: return go<R, N, 2>(a, batch, s);
-    case 3: return go<R, N, 3>(a, batch, s);
-    case 4: return go<R, N, 4>(a, batch, s);
-    case 6: return go<R, N, 6>(a, batch, s);
-    case 10: return go<R, N, 10>(a, batch, s);
+    case 2: return go<R, N, 2>(grad, a, batch, s);
+    case 3: return go<R, N, 3>(grad, a, batch, s);
+    case 4: return go<R, N, 4>(grad, a, batch, s);
+    case 6: return go<R, N, 6>(grad, a, batch, s);
+    case 10: return go<R, N, 10>(grad, a, batch, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <typename R> hipError_t by_size(int nt, int m, const fh::MmArgs<R>& a, int batch, hipStream_t s)
+template <typename R> hipError_t by_size(bool grad, int nt, int m, const fh::MmArgs<R>& a, int batch, hipStream_t s)
 {
     switch (nt) {
-    case 256: return by_modes<R, 256>(m, a, batch, s);
-    case 512: return by_modes<R, 512>(m, a, batch, s);
-    case 1024: return by_modes<R, 1024>(m, a, batch, s);
-    case 2048: return by_modes<R, 2048>(m, a, batch, s);
-    case 4096: return by_modes<R, 4096>(m, a, batch, s);
-    case 8192: return by_modes<R, 8192>(m, a, batch, s);
+    case 256: return by_modes<R, 256>(grad, m, a, batch, s);
+    case 512: return by_modes<R, 512>(grad, m, a, batch, s);
+    case 1024: return by_modes<R, 1024>(grad, m, a, batch, s);
+    case 2048: return by_modes<R, 2048>(grad, m, a, batch, s);
+    case 4096: return by_modes<R, 4096>(grad, m, a, batch, s);
+    case 8192: return by_modes<R, 8192>(grad, m, a, batch, s);
     case 16384:
-        if constexpr (sizeof(R) == 4) return by_modes<R, 16384>(m, a, batch, s);
+        if constexpr (sizeof(R) == 4) return by_modes<R, 16384>(grad, m, a, batch, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
@@ -57,14 +65,26 @@ bool multi_supported(int nt, int m, bool fp64)
 
 int multi_fields(int m) { return 4 * m + (m * m + 1) / 2; }
 
+int multi_grad_fields(int m) { return 9 * m + 2 * ((m * m + 1) / 2); }
+
 hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
 {
-    return by_size<double>(nt, m, a, batch, s);
+    return by_size<double>(false, nt, m, a, batch, s);
 }
 
 hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s)
 {
-    return by_size<float>(nt, m, a, batch, s);
+    return by_size<float>(false, nt, m, a, batch, s);
+}
+
+hipError_t launch_multi_grad(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
+{
+    return by_size<double>(true, nt, m, a, batch, s);
+}
+
+hipError_t launch_multi_grad(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s)
+{
+    return by_size<float>(true, nt, m, a, batch, s);
 }
 
 }  // namespace fh

@@ -1,4 +1,5 @@
-// fh_multi.hpp -- fused multimode RK4IP propagation, one workgroup per solve.
+// fh_multi.hpp -- fused multimode RK4IP propagation and its discrete adjoint,
+// one workgroup per solve.
 //
 // Right-hand side, restating disp_mmf! (simulate_disp_mmf.jl:25-60) for M modes:
 //   u_m   = fft(cis(D_m z) .* y_m)                                    M transforms
@@ -17,10 +18,20 @@
 //   transform pair: ifft(H fft(a + i b)) = R(a) + i R(b).
 // Per right-hand side: 2M + 2 ceil(Q/2) transforms instead of 2M + 2M^2.
 //
+// Pullback of the right-hand side (cotangent gy_i of k_i; real-linear map):
+//   G_i   = fft(-i cis(D_i z) (shock/N) gy_i)                          M transforms
+//   A_q   = sum_{(i,j) -> q} Re(conj(G_i) u_j)        cotangent of h_q
+//   B_q   = (1-fR) A_q + R^T(A_q),  R^T(a) = real(ifft(conj(hR)(-1)^k fft(a)))
+//   C_p   = sum_q Gr[q][p] B_q                        cotangent of Re(u_k conj u_l)
+//   gu_j  = sum_i h_ij G_i + sum_{p=(k,l)} C_p d(Re u_k conj u_l)/du_j
+//   gx_m  = conj(cis(D_m z)) N ifft(gu_m)                             M transforms
+// with u and h of the stage taken from the tape of the segment
+// recomputation; the pair reductions give the forward's transform count.
+//
 // All per-sample data (stage inputs, state, accumulator, time fields, h
-// pairs) lives in a workgroup-private global slice addressed in the cyclic
-// layout, so outside the transforms every thread only touches its own
-// samples and no barrier is needed.
+// pairs, adjoint fields) lives in a workgroup-private global slice addressed
+// in the cyclic layout through one fixed descriptor, so outside the
+// transforms every thread only touches its own samples.
 #pragma once
 #include "fh_device.hpp"
 
@@ -29,7 +40,7 @@ namespace fh {
 template <typename R> struct MmArgs {
     const C2<R>* u0;     // [batch][M][N]
     C2<R>* out;          // [batch][M][N]
-    C2<R>* work;         // [batch][4M + QC][N]
+    C2<R>* work;         // [batch][fields][N]
     const double* disp;  // [M][N]
     const C2<R>* hn;     // [N] hR (-1)^k / N
     const R* shn;        // [N] shock / N
@@ -43,6 +54,17 @@ template <typename R> struct MmArgs {
     int q;  // number of output pairs (M(M+1)/2 or M^2)
     const double* lengths;  // [batch] per-solve fiber length, or null for `length`
     const int* nsteps;      // [batch] per-solve step count, or null for `steps`
+    // adjoint only
+    const C2<R>* seed;          // [batch][M][N] lab-frame Wirtinger seed, or null for the band objective
+    const unsigned char* mask;  // [N] band mask (band energy summed over modes, mmf_cost_sum)
+    int log_cost;
+    C2<R>* lam0;    // [batch][M][N]
+    double* grad;   // [batch][N] sum_m 2 Re(conj(lam0_m) i u0_m)
+    double* cost;   // [batch]
+    C2<R>* ckpt;    // [batch][nseg][M][N]
+    C2<R>* tape_u;  // [batch][4 every][M][N]
+    C2<R>* tape_h;  // [batch][4 every][QC][N]
+    int every, nseg;
 };
 
 template <typename R, int N, int T, int M> struct MmTraits {
@@ -52,12 +74,30 @@ template <typename R, int N, int T, int M> struct MmTraits {
     static constexpr size_t LDS_BYTES = (size_t)LdsSize<N>::value * (PLANAR ? sizeof(R) : 2 * sizeof(R));
     static constexpr int QMAX = M * M;
     static constexpr int QCMAX = (QMAX + 1) / 2;
-    // slice layout: y[M], st[M], ac[M], u[M], hq[QC]
+    // forward slice: y[M], st[M], ac[M], u[M], hq[QC]
     static constexpr int FIELDS = 4 * M + QCMAX;
+    // gradient slice adds lam[M], lacc[M], gx[M], G[M], gu[M], bq[QC]
+    static constexpr int LAM = FIELDS, LACC = LAM + M, GX = LACC + M, GG = GX + M, GU = GG + M, BQ = GU + M;
+    static constexpr int FIELDS_GRAD = BQ + QCMAX;
 };
 
-template <typename R, int N, int T, int M>
-__global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
+// Per-workgroup views shared by the stage and pullback routines.
+template <typename R, int N, int T, int M> struct MmCtx {
+    using V = C2<R>;
+    const MmArgs<R>& a;
+    const Buf& sl;
+    const Buf &disp, &hn, &shn, &tw;
+    int Q, QC;
+    __device__ __forceinline__ Fld fld(int f) const { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; }
+};
+
+// One RK4 stage s of the step from state st at position zs (steps 1-4 of the
+// right-hand side fused with the RK4 bookkeeping of y/st/ac).  With TAPE the
+// stage's time fields u_m and final h pairs are also stored to tape slot
+// `tslot` (tape_u: M fields per slot, tape_h: QC fields per slot).
+template <typename R, int N, int T, int M, bool TAPE>
+__device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, double zs, double h, const Buf& tape_u,
+                                         const Buf& tape_h, uint32_t tslot, C2<R>* v, void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
@@ -66,24 +106,155 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     using CR = Cyc<R, T>;
     constexpr int P = TR::P;
     constexpr int NP = TR::NP;
+    const MmArgs<R>& a = c.a;
+    const uint32_t vo = CV::voff();
+    const uint32_t FB = (uint32_t)(N * sizeof(V));
+    const int Q = c.Q, QC = c.QC;
+    const R* __restrict__ gr = a.gr;
+    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
+    drain_memory();  // re-reads y / st / ac of the previous stage
+    // 1. u_m = fft(cis(D_m zs) y_m)
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld y = c.fld(m), u = c.fld(3 * M + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
+            const V x = y.template ld<V>(vo, CV::soff(k));
+            v[k] = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+        }
+        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) u.st(v[k], vo, CV::soff(k));
+        if constexpr (TAPE) {
+            const Fld tu{tape_u, (tslot * (uint32_t)M + (uint32_t)m) * FB};
+#pragma unroll
+            for (int k = 0; k < P; ++k) tu.st(v[k], vo, CV::soff(k));
+        }
+    }
+    drain_memory();  // re-reads u of step 1
+    // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        V uk[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) uk[m] = c.fld(3 * M + m).template ld<V>(vo, CV::soff(k));
+        R b[NP];
+        {
+            int p = 0;
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = i; j < M; ++j) b[p++] = uk[i].x * uk[j].x + uk[i].y * uk[j].y;
+        }
+#pragma unroll 1
+        for (int cc = 0; cc < QC; ++cc) {
+            R d0 = 0, d1 = 0;
+            const R* g0 = gr + (size_t)(2 * cc) * NP;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) d0 += g0[p] * b[p];
+            if (2 * cc + 1 < Q) {
+                const R* g1 = g0 + NP;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) d1 += g1[p] * b[p];
+            }
+            c.fld(4 * M + cc).st(V{d0, d1}, vo, CV::soff(k));
+        }
+    }
+    drain_memory();  // re-reads hq of step 2
+    // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))
+#pragma unroll 1
+    for (int cc = 0; cc < QC; ++cc) {
+        const Fld hq = c.fld(4 * M + cc);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = hq.template ld<V>(vo, CV::soff(k));
+        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = cmul(v[k], c.hn.template ld<V>(vo, CV::soff(k)));
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+        const Fld th{tape_h, (tslot * (uint32_t)QC + (uint32_t)cc) * FB};
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V d = hq.template ld<V>(vo, CV::soff(k));
+            const V hv = V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y};
+            hq.st(hv, vo, CV::soff(k));
+            if constexpr (TAPE) th.st(hv, vo, CV::soff(k));
+        }
+    }
+    drain_memory();  // re-reads hq of step 3
+    // 4. per mode: eta_i = sum_j h_ij u_j; k_i; RK4 bookkeeping
+#pragma unroll 1
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            V acc = V{(R)0, (R)0};
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const int q = a.qidx[i * M + j];
+                const V hp = c.fld(4 * M + (q >> 1)).template ld<V>(vo, CV::soff(k));
+                const R hij = (q & 1) ? hp.y : hp.x;
+                const V uj = c.fld(3 * M + j).template ld<V>(vo, CV::soff(k));
+                acc = V{acc.x + hij * uj.x, acc.y + hij * uj.y};
+            }
+            v[k] = acc;
+        }
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+        const Fld y = c.fld(i), st = c.fld(M + i), ac = c.fld(2 * M + i);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+            const V e = V{v[k].x * sc, v[k].y * sc};
+            const V kk = V{sn * e.x - cs * e.y, sn * e.y + cs * e.x};
+            const V x = st.template ld<V>(vo, CV::soff(k));
+            V cc = s != 0 ? ac.template ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
+            V yn;
+            if (s == 0) {
+                cc = kk;
+                yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+            } else if (s == 1) {
+                cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
+                yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+            } else if (s == 2) {
+                cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
+                yn = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
+            } else {
+                const V t = V{cc.x + kk.x, cc.y + kk.y};
+                yn = V{x.x + h6 * t.x, x.y + h6 * t.y};
+                st.st(yn, vo, CV::soff(k));
+            }
+            if (s != 3) ac.st(cc, vo, CV::soff(k));
+            y.st(yn, vo, CV::soff(k));
+        }
+    }
+}
+
+template <typename R, int N, int T, int M>
+__global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = TR::P;
     __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
     const size_t sol = blockIdx.x;
-    const int Q = a.q, QC = (a.q + 1) / 2;
     // One descriptor for the whole workgroup slice, fixed for the kernel's
     // lifetime; field f is addressed through the per-lane offset
     // vo + f * N * sizeof(V) (see Fld).
     V* slice = a.work + sol * (size_t)TR::FIELDS * N;
     const Buf sl(slice, (uint32_t)((size_t)TR::FIELDS * N * sizeof(V)));
-    auto fld = [&](int f) { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; };
     const Buf disp(a.disp, M * N * 8), hn(a.hn, N * sizeof(V)), shn(a.shn, N * sizeof(R));
     const Buf tw(a.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
     const Buf in(a.u0 + sol * M * N, M * N * sizeof(V));
-    const R* __restrict__ gr = a.gr;
+    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, a.q, (a.q + 1) / 2};
     const uint32_t vo = CV::voff();
 
     // y = st = u0
     for (int m = 0; m < M; ++m) {
-        const Fld y = fld(m), st = fld(M + m);
+        const Fld y = c.fld(m), st = c.fld(M + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
@@ -94,138 +265,400 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     const double Lb = a.lengths ? a.lengths[sol] : a.length;
     const int Sb = a.nsteps ? a.nsteps[sol] : a.steps;
     const double h = a.lengths ? Lb / Sb : a.h;
-    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
     V v[P];
+    const Buf none(a.u0, 0);
     for (int step = 0; step < Sb; ++step) {
         const double z = step * h;
 #pragma unroll 1
         for (int s = 0; s < 4; ++s) {
             const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
-            drain_memory();  // re-reads y / st / ac of the previous stage
-            // 1. u_m = fft(cis(D_m zs) y_m)
-#pragma unroll 1
-            for (int m = 0; m < M; ++m) {
-                const Fld y = fld(m), u = fld(3 * M + m);
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    R c, sn;
-                    cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, c, sn);
-                    const V x = y.template ld<V>(vo, CV::soff(k));
-                    v[k] = V{x.x * c - x.y * sn, x.x * sn + x.y * c};
-                }
-                block_fft<R, N, T, -1, TR::PLANAR>(v, tw, lds);
-#pragma unroll
-                for (int k = 0; k < P; ++k) u.st(v[k], vo, CV::soff(k));
-            }
-            drain_memory();  // re-reads u of step 1
-            // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p
-#pragma unroll 1
-            for (int k = 0; k < P; ++k) {
-                V uk[M];
-#pragma unroll
-                for (int m = 0; m < M; ++m) uk[m] = fld(3 * M + m).template ld<V>(vo, CV::soff(k));
-                R b[NP];
-                {
-                    int p = 0;
-#pragma unroll
-                    for (int i = 0; i < M; ++i)
-#pragma unroll
-                        for (int j = i; j < M; ++j) b[p++] = uk[i].x * uk[j].x + uk[i].y * uk[j].y;
-                }
-#pragma unroll 1
-                for (int c = 0; c < QC; ++c) {
-                    R d0 = 0, d1 = 0;
-                    const R* g0 = gr + (size_t)(2 * c) * NP;
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) d0 += g0[p] * b[p];
-                    if (2 * c + 1 < Q) {
-                        const R* g1 = g0 + NP;
-#pragma unroll
-                        for (int p = 0; p < NP; ++p) d1 += g1[p] * b[p];
-                    }
-                    fld(4 * M + c).st(V{d0, d1}, vo, CV::soff(k));
-                }
-            }
-            drain_memory();  // re-reads hq of step 2
-            // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))
-#pragma unroll 1
-            for (int c = 0; c < QC; ++c) {
-                const Fld hq = fld(4 * M + c);
-#pragma unroll
-                for (int k = 0; k < P; ++k) v[k] = hq.template ld<V>(vo, CV::soff(k));
-                block_fft<R, N, T, -1, TR::PLANAR>(v, tw, lds);
-#pragma unroll
-                for (int k = 0; k < P; ++k) v[k] = cmul(v[k], hn.ld<V>(vo, CV::soff(k)));
-                block_fft<R, N, T, +1, TR::PLANAR>(v, tw, lds);
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    const V d = hq.template ld<V>(vo, CV::soff(k));
-                    hq.st(V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y}, vo, CV::soff(k));
-                }
-            }
-            drain_memory();  // re-reads hq of step 3
-            // 4. per mode: eta_i = sum_j h_ij u_j; k_i; RK4 bookkeeping
-#pragma unroll 1
-            for (int i = 0; i < M; ++i) {
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    V acc = V{(R)0, (R)0};
-#pragma unroll
-                    for (int j = 0; j < M; ++j) {
-                        const int q = a.qidx[i * M + j];
-                        const V hp = fld(4 * M + (q >> 1)).template ld<V>(vo, CV::soff(k));
-                        const R hij = (q & 1) ? hp.y : hp.x;
-                        const V uj = fld(3 * M + j).template ld<V>(vo, CV::soff(k));
-                        acc = V{acc.x + hij * uj.x, acc.y + hij * uj.y};
-                    }
-                    v[k] = acc;
-                }
-                block_fft<R, N, T, +1, TR::PLANAR>(v, tw, lds);
-                const Fld y = fld(i), st = fld(M + i), ac = fld(2 * M + i);
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    R c, sn;
-                    cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, c, sn);
-                    const R sc = shn.ld<R>(CR::voff(), CR::soff(k));
-                    const V e = V{v[k].x * sc, v[k].y * sc};
-                    const V kk = V{sn * e.x - c * e.y, sn * e.y + c * e.x};
-                    const V x = st.template ld<V>(vo, CV::soff(k));
-                    V cc = s != 0 ? ac.template ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
-                    V yn;
-                    if (s == 0) {
-                        cc = kk;
-                        yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-                    } else if (s == 1) {
-                        cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
-                        yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-                    } else if (s == 2) {
-                        cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
-                        yn = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
-                    } else {
-                        const V t = V{cc.x + kk.x, cc.y + kk.y};
-                        yn = V{x.x + h6 * t.x, x.y + h6 * t.y};
-                        st.st(yn, vo, CV::soff(k));
-                    }
-                    if (s != 3) ac.st(cc, vo, CV::soff(k));
-                    y.st(yn, vo, CV::soff(k));
-                }
-            }
+            mm_stage<R, N, T, M, false>(c, s, zs, h, none, none, 0, v, lds);
         }
     }
+    drain_memory();
     int bad = 0;
     const Buf out(a.out + sol * M * N, M * N * sizeof(V));
     for (int m = 0; m < M; ++m) {
-        const Fld st = fld(M + m);
+        const Fld st = c.fld(M + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            R c, sn;
-            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, c, sn);
+            R cs, sn;
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, cs, sn);
             const V x = st.template ld<V>(vo, CV::soff(k));
-            const V o = V{x.x * c - x.y * sn, x.x * sn + x.y * c};
+            const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
             bad |= !(isfinite(o.x) && isfinite(o.y));
             out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
         }
     }
+    if (bad) atomicOr(a.status + sol, 1);
+}
+
+// Pullback of one right-hand-side evaluation at zs.  The cotangent of stage
+// output k_i is cl lam_i + cg gx_i (gx_i: the previous pullback's output, not
+// read when cg == 0); on exit gx_i holds the cotangent of the stage input.
+// u and h of the stage come from tape slot `tslot`.
+template <typename R, int N, int T, int M>
+__device__ __forceinline__ void mm_vjp(const MmCtx<R, N, T, M>& c, double zs, R cl, R cg, const Buf& tape_u,
+                                       const Buf& tape_h, uint32_t tslot, C2<R>* v, void* lds)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    using CR = Cyc<R, T>;
+    constexpr int P = TR::P;
+    constexpr int NP = TR::NP;
+    constexpr int QMAX = TR::QMAX;
+    constexpr int QCMAX = TR::QCMAX;
+    const MmArgs<R>& a = c.a;
+    const uint32_t vo = CV::voff();
+    const uint32_t FB = (uint32_t)(N * sizeof(V));
+    const int Q = c.Q, QC = c.QC;
+    const bool sym = Q != M * M;  // output pairs i <= j (h_ij = h_ji)
+    const R* __restrict__ gr = a.gr;
+    auto tu = [&](int m) { return Fld{tape_u, (tslot * (uint32_t)M + (uint32_t)m) * FB}; };
+    auto th = [&](int cc) { return Fld{tape_h, (tslot * (uint32_t)QC + (uint32_t)cc) * FB}; };
+    drain_memory();  // lam / gx were stored by the previous stage
+    // V1: G_i = fft(-i cis(D_i zs) (shock/N) gy_i)
+#pragma unroll 1
+    for (int i = 0; i < M; ++i) {
+        const Fld lam = c.fld(TR::LAM + i), gx = c.fld(TR::GX + i), G = c.fld(TR::GG + i);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V l = lam.template ld<V>(vo, CV::soff(k));
+            V g = V{cl * l.x, cl * l.y};
+            if (cg != (R)0) {
+                const V x = gx.template ld<V>(vo, CV::soff(k));
+                g = V{g.x + cg * x.x, g.y + cg * x.y};
+            }
+            R cs, sn;
+            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+            const V e = V{g.x * sc, g.y * sc};
+            const V w = V{e.x * cs - e.y * sn, e.x * sn + e.y * cs};
+            v[k] = V{w.y, -w.x};  // -i w
+        }
+        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) G.st(v[k], vo, CV::soff(k));
+    }
+    drain_memory();  // re-reads G
+    // V2: gu_j = sum_i h_ij G_i; A_q in output-pair order, packed into bq
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        V G[M], u[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            G[m] = c.fld(TR::GG + m).template ld<V>(vo, CV::soff(k));
+            u[m] = tu(m).template ld<V>(vo, CV::soff(k));
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            V g = V{(R)0, (R)0};
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const int q = a.qidx[i * M + j];
+                const V hp = th(q >> 1).template ld<V>(vo, CV::soff(k));
+                const R hij = (q & 1) ? hp.y : hp.x;
+                g = V{g.x + hij * G[i].x, g.y + hij * G[i].y};
+            }
+            c.fld(TR::GU + j).st(g, vo, CV::soff(k));
+        }
+        R A[QMAX + 1];
+        A[QMAX] = (R)0;
+        if (sym) {
+            int q = 0;
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = i; j < M; ++j) {
+                    R s = G[i].x * u[j].x + G[i].y * u[j].y;
+                    if (j != i) s += G[j].x * u[i].x + G[j].y * u[i].y;
+                    A[q++] = s;
+                }
+#pragma unroll
+            for (int r = TR::NP; r < QMAX; ++r) A[r] = (R)0;
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) A[i * M + j] = G[i].x * u[j].x + G[i].y * u[j].y;
+        }
+#pragma unroll
+        for (int cc = 0; cc < QCMAX; ++cc)
+            if (cc < QC) c.fld(TR::BQ + cc).st(V{A[2 * cc], 2 * cc + 1 < Q ? A[2 * cc + 1] : (R)0}, vo, CV::soff(k));
+    }
+    drain_memory();  // re-reads bq
+    // V3: B_q = (1-fR) A_q + R^T(A_q) on packed pairs
+#pragma unroll 1
+    for (int cc = 0; cc < QC; ++cc) {
+        const Fld bq = c.fld(TR::BQ + cc);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = bq.template ld<V>(vo, CV::soff(k));
+        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = cmul(v[k], cconj(c.hn.template ld<V>(vo, CV::soff(k))));
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V d = bq.template ld<V>(vo, CV::soff(k));
+            bq.st(V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y}, vo, CV::soff(k));
+        }
+    }
+    drain_memory();  // re-reads bq and gu
+    // V4: C_p = sum_q Gr[q][p] B_q; gu += C_p d(Re u_k conj u_l)/du
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        R C[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) C[p] = (R)0;
+#pragma unroll 1
+        for (int cc = 0; cc < QC; ++cc) {
+            const V b = c.fld(TR::BQ + cc).template ld<V>(vo, CV::soff(k));
+            const R* g0 = gr + (size_t)(2 * cc) * NP;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) C[p] += g0[p] * b.x;
+            if (2 * cc + 1 < Q) {
+                const R* g1 = g0 + NP;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) C[p] += g1[p] * b.y;
+            }
+        }
+        V u[M], gu[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            u[m] = tu(m).template ld<V>(vo, CV::soff(k));
+            gu[m] = c.fld(TR::GU + m).template ld<V>(vo, CV::soff(k));
+        }
+        int p = 0;
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk)
+#pragma unroll
+            for (int ll = kk; ll < M; ++ll) {
+                const R cp = C[p++];
+                if (kk == ll) {
+                    gu[kk] = V{gu[kk].x + 2 * cp * u[kk].x, gu[kk].y + 2 * cp * u[kk].y};
+                } else {
+                    gu[kk] = V{gu[kk].x + cp * u[ll].x, gu[kk].y + cp * u[ll].y};
+                    gu[ll] = V{gu[ll].x + cp * u[kk].x, gu[ll].y + cp * u[kk].y};
+                }
+            }
+#pragma unroll
+        for (int m = 0; m < M; ++m) c.fld(TR::GU + m).st(gu[m], vo, CV::soff(k));
+    }
+    drain_memory();  // re-reads gu
+    // V5: gx_m = conj(cis(D_m zs)) N ifft(gu_m)
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld gu = c.fld(TR::GU + m), gx = c.fld(TR::GX + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = gu.template ld<V>(vo, CV::soff(k));
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
+            gx.st(V{cs * v[k].x + sn * v[k].y, cs * v[k].y - sn * v[k].x}, vo, CV::soff(k));
+        }
+    }
+}
+
+// Workgroup FP64 sum (fixed tree order) through the LDS buffer.
+template <int T> __device__ __forceinline__ double mm_block_sum(double x, void* lds_raw)
+{
+    double* red = reinterpret_cast<double*>(lds_raw);
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = x;
+    __syncthreads();
+#pragma unroll 1
+    for (int w = T / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Forward with checkpoints, terminal seed (band energy summed over modes,
+// mmf_cost_sum of mmf_cost.jl:49-62, or a caller seed), backward sweep of the
+// discrete adjoint with per-segment recomputation, lam0 and the shared-phase
+// gradient sum_m 2 Re(conj(lam0_m) i u0_m) in FP64
+// (mmf_raman_optimization.jl:268-271).
+template <typename R, int N, int T, int M>
+__global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = TR::P;
+    __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
+    const size_t sol = blockIdx.x;
+    const uint32_t FB = (uint32_t)(N * sizeof(V));
+    V* slice = a.work + sol * (size_t)TR::FIELDS_GRAD * N;
+    const Buf sl(slice, (uint32_t)((size_t)TR::FIELDS_GRAD * N * sizeof(V)));
+    const Buf disp(a.disp, M * N * 8), hn(a.hn, N * sizeof(V)), shn(a.shn, N * sizeof(R));
+    const Buf tw(a.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf in(a.u0 + sol * M * N, M * N * sizeof(V));
+    const int Q = a.q, QC = (a.q + 1) / 2;
+    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, Q, QC};
+    const Buf ckpt(a.ckpt + sol * (size_t)a.nseg * M * N, (uint32_t)((size_t)a.nseg * M * N * sizeof(V)));
+    const Buf tape_u(a.tape_u + sol * (size_t)4 * a.every * M * N,
+                     (uint32_t)((size_t)4 * a.every * M * N * sizeof(V)));
+    const Buf tape_h(a.tape_h + sol * (size_t)4 * a.every * QC * N,
+                     (uint32_t)((size_t)4 * a.every * QC * N * sizeof(V)));
+    const Buf none(a.u0, 0);
+    const uint32_t vo = CV::voff();
+    // per-solve length and step count (the plan's steps bound the checkpoint capacity)
+    const double Lb = a.lengths ? a.lengths[sol] : a.length;
+    const int Sb = a.nsteps ? a.nsteps[sol] : a.steps;
+    const double h = a.lengths ? Lb / Sb : a.h;
+    const int nseg = (Sb + a.every - 1) / a.every;
+    V v[P];
+    for (int m = 0; m < M; ++m) {
+        const Fld y = c.fld(m), st = c.fld(M + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            y.st(x, vo, CV::soff(k));
+            st.st(x, vo, CV::soff(k));
+        }
+    }
+    // forward with checkpoints of the state
+    for (int step = 0; step < Sb; ++step) {
+        if (step % a.every == 0) {
+            drain_memory();
+            for (int m = 0; m < M; ++m) {
+                const Fld st = c.fld(M + m);
+                const Fld ck{ckpt, ((uint32_t)(step / a.every) * (uint32_t)M + (uint32_t)m) * FB};
+#pragma unroll
+                for (int k = 0; k < P; ++k) ck.st(st.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+            }
+        }
+        const double z = step * h;
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s)
+            mm_stage<R, N, T, M, false>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, none, none, 0, v, lds);
+    }
+    drain_memory();
+    // output and terminal seed (band energy over all modes)
+    int bad = 0;
+    const Buf out(a.out + sol * M * N, M * N * sizeof(V));
+    double etot = 0.0, eband = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const Fld st = c.fld(M + m), lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, cs, sn);
+            const V x = st.template ld<V>(vo, CV::soff(k));
+            const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+            bad |= !(isfinite(o.x) && isfinite(o.y));
+            out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
+            const double e = (double)o.x * o.x + (double)o.y * o.y;
+            etot += e;
+            if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+        }
+    }
+    double J = 0.0, scale = 0.0;
+    if (!a.seed) {
+        etot = mm_block_sum<T>(etot, lds);
+        eband = mm_block_sum<T>(eband, lds);
+        J = eband / etot;
+        scale = 1.0 / etot;
+        if (a.log_cost) scale *= J <= 1e-15 ? 0.0 : 10.0 / (2.302585092994045684 * J);
+        if (threadIdx.x == 0) a.cost[sol] = a.log_cost ? 10.0 * log10(fmax(J, 1e-15)) : J;
+    }
+    drain_memory();
+    // lam = cis(-D L) seed
+    const Buf sd(a.seed ? a.seed + sol * M * N : a.u0, a.seed ? (uint32_t)(M * N * sizeof(V)) : 0u);
+    for (int m = 0; m < M; ++m) {
+        const Fld lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            V s;
+            if (a.seed) {
+                s = sd.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            } else {
+                const V o = lam.template ld<V>(vo, CV::soff(k));
+                const R f = (R)(((a.mask[threadIdx.x + T * k] ? 1.0 : 0.0) - J) * scale);
+                s = V{o.x * f, o.y * f};
+            }
+            R cs, sn;
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, cs, sn);
+            lam.st(V{cs * s.x + sn * s.y, cs * s.y - sn * s.x}, vo, CV::soff(k));
+        }
+    }
+    const R c6 = (R)(h / 6), c3 = (R)(h / 3), c2 = (R)(h / 2), c1 = (R)h;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * a.every;
+        const int s1 = s0 + a.every < Sb ? s0 + a.every : Sb;
+        drain_memory();
+        for (int m = 0; m < M; ++m) {
+            const Fld ck{ckpt, ((uint32_t)sg * (uint32_t)M + (uint32_t)m) * FB};
+            const Fld y = c.fld(m), st = c.fld(M + m);
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const V x = ck.template ld<V>(vo, CV::soff(k));
+                y.st(x, vo, CV::soff(k));
+                st.st(x, vo, CV::soff(k));
+            }
+        }
+        for (int step = s0; step < s1; ++step) {
+            const double z = step * h;
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s)
+                mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
+                                           (uint32_t)(4 * (step - s0) + s), v, lds);
+        }
+        for (int step = s1 - 1; step >= s0; --step) {
+            const double z = step * h;
+#pragma unroll 1
+            for (int s = 3; s >= 0; --s) {
+                const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
+                const R cl = (s == 0 || s == 3) ? c6 : c3;
+                const R cg = s == 3 ? (R)0 : (s == 2 ? c1 : c2);
+                mm_vjp<R, N, T, M>(c, zs, cl, cg, tape_u, tape_h, (uint32_t)(4 * (step - s0) + s), v, lds);
+                drain_memory();
+                for (int m = 0; m < M; ++m) {
+                    const Fld lam = c.fld(TR::LAM + m), lacc = c.fld(TR::LACC + m), gx = c.fld(TR::GX + m);
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const V g = gx.template ld<V>(vo, CV::soff(k));
+                        const V base = s == 3 ? lam.template ld<V>(vo, CV::soff(k)) : lacc.template ld<V>(vo, CV::soff(k));
+                        const V nc = V{base.x + g.x, base.y + g.y};
+                        if (s == 0)
+                            lam.st(nc, vo, CV::soff(k));
+                        else
+                            lacc.st(nc, vo, CV::soff(k));
+                    }
+                }
+            }
+        }
+    }
+    drain_memory();
+    // lam0 and the shared-phase gradient in FP64
+    const Buf l0(a.lam0 + sol * M * N, M * N * sizeof(V));
+    const Buf gb(a.grad + sol * N, N * sizeof(double));
+    double g[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) g[k] = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const Fld lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V l = lam.template ld<V>(vo, CV::soff(k));
+            const V u = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            l0.st(l, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            g[k] += 2.0 * ((double)l.y * (double)u.x - (double)l.x * (double)u.y);
+            bad |= !(isfinite(l.x) && isfinite(l.y));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) gb.st(g[k], CD::voff(), CD::soff(k));
     if (bad) atomicOr(a.status + sol, 1);
 }
 

@@ -134,6 +134,8 @@ struct fh_server_s {
 
 namespace {
 
+int mm_grad_cap(const fh_server_s* S) { return S->cfg.mm_grad_max_steps > 0 ? S->cfg.mm_grad_max_steps : 2000; }
+
 void finish(fh_server_s* S, int64_t id, int code)
 {
     Req& r = S->reqs[id];
@@ -171,6 +173,14 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
     d.precision = S->cfg.precision;
     d.scheme = o.modes == 1 ? FH_SCHEME_TSIT5 : FH_SCHEME_RK4IP_FIXED;
     d.steps = 1;  // multimode: per-solve step counts override
+    d.checkpoint_every = 50;
+    if (o.modes > 1 && o.grad) {
+        // step capacity; checkpoint spacing minimising checkpoints (cap/every M fields)
+        // plus tape (4 every (M + QC) fields) per solve
+        const int cap = mm_grad_cap(S), M = o.modes, qc = (M * (M + 1) / 2 + 1) / 2;
+        d.steps = cap;
+        d.checkpoint_every = std::max(1, (int)std::lround(std::sqrt((double)cap * M / (4.0 * (M + qc)))));
+    }
     d.max_batch = S->cfg.max_batch;
     d.length_m = 1e-3;  // per-solve lengths override; the phase bound is checked per request
     d.dispersion = o.disp.data();
@@ -178,7 +188,6 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
     d.raman = o.raman.data();
     d.shock = o.shock.data();
     d.one_m_fr = o.one_m_fr;
-    d.checkpoint_every = 50;
     d.band_mask = o.grad ? o.mask.data() : nullptr;
     d.rtol = S->cfg.rtol;
     d.atol = S->cfg.atol;
@@ -369,8 +378,9 @@ int validate(const fh_server_s* S, const fh_request* r, double* dmax)
         return fhi::set_error(FH_EINVAL, "operator, launch or output pointer is NULL");
     if (!(std::isfinite(r->length_m) && r->length_m > 0)) return fhi::set_error(FH_EINVAL, "fiber length must be positive and finite");
     if (r->kind == FH_REQ_GRADIENT) {
-        if (r->modes > 1) return fhi::set_error(FH_EUNSUPPORTED, "multimode gradients are not offered by the server");
         if (!r->band_mask) return fhi::set_error(FH_EINVAL, "gradient requests need a band mask");
+        if (r->modes > 1 && std::ceil(r->length_m / S->cfg.multimode_step_m) > mm_grad_cap(S))
+            return fhi::set_error(FH_EUNSUPPORTED, "multimode gradient needs more steps than the server's capacity");
     }
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;
     if (r->modes == 1 ? !fh::tsit5_supported(r->nt, 1) : !fh::multi_supported(r->nt, r->modes, fp64))

@@ -55,6 +55,7 @@ struct fh_plan_s {
     int64_t total_bytes = 0;
     double gamma0 = 0.0;        // gamma[1,1,1,1] of a single-mode plan
     bool adaptive = false;      // FH_SCHEME_TSIT5
+    bool gamma_full_sym = true; // gamma invariant under i<->j, j<->k, k<->l (sensitivity_disp_mmf.jl:103-112)
     int max_steps = 0;
     DevBuf hs, zs, sstats;      // adaptive: step sizes, positions [max_batch][max_steps]; counters [max_batch][3]
 };
@@ -80,6 +81,13 @@ void dev_free(DevBuf& b)
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+}
+
+// Free a plan buffer and take it off the plan's byte count.
+void total_release(fh_plan p, DevBuf& b)
+{
+    p->total_bytes -= (int64_t)b.bytes;
+    dev_free(b);
 }
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -323,6 +331,18 @@ template <typename R> int upload_tables(fh_plan p)
                 }
         }
         p->q = (int)outp.size();
+        const double tol = 1e-12 * std::max(1.0, gmax);
+        auto close = [&](double x, double y) { return std::fabs(x - y) <= tol + 1e-12 * std::fabs(y); };
+        for (int i = 0; i < M && p->gamma_full_sym; ++i)
+            for (int j = 0; j < M && p->gamma_full_sym; ++j)
+                for (int k = 0; k < M && p->gamma_full_sym; ++k)
+                    for (int l = 0; l < M; ++l) {
+                        const double g = G(i, j, k, l);
+                        if (!close(g, G(j, i, k, l)) || !close(g, G(i, k, j, l)) || !close(g, G(i, j, l, k))) {
+                            p->gamma_full_sym = false;
+                            break;
+                        }
+                    }
         std::vector<R> gr(outp.size() * inp.size());
         for (size_t qq = 0; qq < outp.size(); ++qq)
             for (size_t pp = 0; pp < inp.size(); ++pp) {
@@ -386,28 +406,76 @@ template <typename R> fh::SmArgs<R> base_args(fh_plan p)
 template <typename R> int ensure_adjoint_buffers(fh_plan p)
 {
     if (p->ckpt.p) return FH_OK;
-    const size_t nt = (size_t)p->d.nt, mb = (size_t)p->d.max_batch, V = 2 * sizeof(R);
+    const size_t nt = (size_t)p->d.nt, mb = (size_t)p->d.max_batch, V = 2 * sizeof(R), M = (size_t)p->d.modes;
     int rc;
-    if ((rc = dev_alloc(p, p->ckpt, V * nt * (size_t)p->nseg * mb))) return rc;
+    if (M > 1) {  // the gradient kernel's slice holds the adjoint fields too
+        total_release(p, p->work);
+        if ((rc = dev_alloc(p, p->work, V * (size_t)fh::multi_grad_fields((int)M) * nt * mb))) return rc;
+    }
+    if ((rc = dev_alloc(p, p->ckpt, V * nt * M * (size_t)p->nseg * mb))) return rc;
     const size_t stages = p->adaptive ? 6 : 4;
-    if ((rc = dev_alloc(p, p->tape_u, V * nt * stages * (size_t)p->every * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_h, sizeof(R) * nt * stages * (size_t)p->every * mb))) return rc;
-    if ((rc = dev_alloc(p, p->g_lam0, V * nt * mb))) return rc;
+    // tape per stage: u of every mode; h as one real field (single mode) or ceil(Q/2) packed pairs
+    const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((p->q + 1) / 2);
+    if ((rc = dev_alloc(p, p->tape_u, V * nt * M * stages * (size_t)p->every * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, hbytes * stages * (size_t)p->every * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_lam0, V * nt * M * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;
-    if ((rc = dev_alloc(p, p->g_seed, V * nt * mb))) return rc;
+    if ((rc = dev_alloc(p, p->g_seed, V * nt * M * mb))) return rc;
     return FH_OK;
 }
 
 template <typename R>
 int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
-                    double* d_grad, double* d_cost, hipStream_t s, const double* d_lengths = nullptr)
+                    double* d_grad, double* d_cost, hipStream_t s, const double* d_lengths = nullptr,
+                    const int* d_nsteps = nullptr)
 {
-    if (p->d.modes != 1) return fail(FH_EUNSUPPORTED, "the adjoint kernel of this build is single-mode");
     if (!d_seed && !p->has_mask)
         return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
+    if (p->d.modes > 1) {
+        if (!p->gamma_full_sym)
+            return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
+        if (!d_lengths != !d_nsteps) return fail(FH_EINVAL, "multimode per-solve lengths need per-solve step counts");
+        if (p->adaptive) return fail(FH_EUNSUPPORTED, "multimode gradients use the fixed-step scheme");
+    }
     int rc = ensure_adjoint_buffers<R>(p);
     if (rc) return rc;
+    if (p->d.modes > 1) {
+        const fh_problem_desc& d = p->d;
+        fh::MmArgs<R> m{};
+        m.u0 = (const fh::C2<R>*)d_u0;
+        m.out = (fh::C2<R>*)d_uL;
+        m.work = (fh::C2<R>*)p->work.p;
+        m.disp = (const double*)p->disp.p;
+        m.hn = (const fh::C2<R>*)p->hn.p;
+        m.shn = (const R*)p->shn.p;
+        m.tw = (const fh::C2<R>*)p->tw.p;
+        m.gr = (const R*)p->gr.p;
+        m.qidx = (const int*)p->qidx.p;
+        m.status = (int*)p->status.p;
+        m.one_m_fr = (R)d.one_m_fr;
+        m.h = d.length_m / d.steps;
+        m.length = d.length_m;
+        m.steps = d.steps;
+        m.q = p->q;
+        m.seed = (const fh::C2<R>*)d_seed;
+        m.mask = (const unsigned char*)p->mask.p;
+        m.log_cost = d.log_cost;
+        m.lam0 = (fh::C2<R>*)d_lam0;
+        m.grad = d_grad;
+        m.cost = d_cost;
+        m.ckpt = (fh::C2<R>*)p->ckpt.p;
+        m.tape_u = (fh::C2<R>*)p->tape_u.p;
+        m.tape_h = (fh::C2<R>*)p->tape_h.p;
+        m.every = p->every;
+        m.nseg = p->nseg;
+        m.lengths = d_lengths;
+        m.nsteps = d_nsteps;  // each at most the plan's steps (checkpoint capacity)
+        FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+        hipError_t e = fh::launch_multi_grad(d.nt, d.modes, m, batch, s);
+        if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode gradient launch: ") + hipGetErrorString(e));
+        return FH_OK;
+    }
     fh::SmArgs<R> a = base_args<R>(p);
     a.u0 = (const fh::C2<R>*)d_u0;
     a.seed = (const fh::C2<R>*)d_seed;
@@ -553,9 +621,9 @@ int enqueue_batch(fh_plan p, int batch, bool grad, const double* d_lengths, cons
     if (grad) {
         if ((rc = plan_prepare_gradient(p))) return rc;
         rc = fp64 ? launch_gradient<double>(p, batch, p->u_in.p, nullptr, p->u_out.p, p->g_lam0.p,
-                                            (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths)
+                                            (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths, d_nsteps)
                   : launch_gradient<float>(p, batch, p->u_in.p, nullptr, p->u_out.p, p->g_lam0.p,
-                                           (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths);
+                                           (double*)p->g_grad.p, (double*)p->g_cost.p, s, d_lengths, d_nsteps);
     } else {
         rc = fp64 ? launch_forward<double>(p, batch, p->u_in.p, p->u_out.p, s, d_lengths, d_nsteps)
                   : launch_forward<float>(p, batch, p->u_in.p, p->u_out.p, s, d_lengths, d_nsteps);

@@ -92,6 +92,8 @@ class ServerConfig(ctypes.Structure):
         ("rtol", ctypes.c_double),
         ("atol", ctypes.c_double),
         ("multimode_step_m", ctypes.c_double),
+        ("mm_grad_max_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

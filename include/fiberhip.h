@@ -204,8 +204,9 @@ int fh_step_sizes(fh_plan plan, int32_t batch, double *hs);
  * its own inputs, so it is the same alone and inside any batch.
  * Single-mode requests run the adaptive Tsit5 scheme (config rtol/atol) with
  * their own length; multimode requests run the fixed-step RK4IP with
- * ceil(length / multimode_step_m) steps (at least 16); multimode gradients are
- * not offered (FH_EUNSUPPORTED).  No counterpart exists in the reference (its
+ * ceil(length / multimode_step_m) steps (at least 16); multimode gradients
+ * are offered up to mm_grad_max_steps steps (their checkpoint capacity) for
+ * fully permutation-symmetric gamma.  No counterpart exists in the reference (its
  * solves run one at a time in the calling Julia task); callers are batch
  * optimisation drivers and services that issue many solves. */
 
@@ -219,6 +220,8 @@ typedef struct {
     int32_t max_steps;           /* adaptive accepted-step limit (0 = 10000)       */
     double rtol, atol;           /* adaptive tolerances of single-mode requests    */
     double multimode_step_m;     /* fixed step of multimode requests               */
+    int32_t mm_grad_max_steps;   /* step capacity of multimode gradients (0 = 2000)*/
+    int32_t reserved;
 } fh_server_config;
 
 enum { FH_REQ_FORWARD = 0, FH_REQ_GRADIENT = 1 };

@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 
 def _requests():
     """A mixed list of (problem, kind, length, launch): single-mode SMF-28 at
-    three grid sizes (forward and band-objective gradient) and 3-mode GRIN
-    forwards, with uneven lengths and launch phases."""
+    three grid sizes and 3-mode GRIN, each forward and band-objective
+    gradient, with uneven lengths and launch phases."""
     import fiberhip
     from fiberhip import grin
 
@@ -28,7 +28,8 @@ def _requests():
     out = []
     for i in range(24):
         if i % 4 == 3:
-            out.append((mm, "forward", float(rng.uniform(0.002, 0.01)), mm.uω0 * np.exp(1j * rng.uniform(0, 6.28))))
+            kind = "gradient" if i % 8 == 7 else "forward"
+            out.append((mm, kind, float(rng.uniform(0.002, 0.01)), mm.uω0 * np.exp(1j * rng.uniform(0, 6.28))))
             continue
         nt = (512, 1024, 2048)[i % 3]
         p = smf[nt]
@@ -91,7 +92,9 @@ def test_results_same_alone_and_in_the_mix(gpu_context, precision):
         else:
             steps = max(16, int(np.ceil(length / 2.5e-4)))
             assert got["accepted"] == steps
-            ref = fiberhip.Plan.from_problem(q, steps=steps, precision=precision).forward(u0)[0]
+            plan = fiberhip.Plan.from_problem(q, steps=steps, precision=precision,
+                                              band_mask=fiberhip.frequency_band_mask(p, (-16.0, -10.0)))
+            ref = plan.gradient(u0)["uL"] if kind == "gradient" else plan.forward(u0)[0]
         assert np.array_equal(ref, got["uL"])
 
 
@@ -115,8 +118,8 @@ def test_memory_budget_admission_and_rejections(gpu_context):
         for a, b in zip(got, small):
             assert np.array_equal(a["uL"], b["uL"])
         mm = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.01, energy_j=1e-9, nx=96, window_um=60.0)
-        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
-            tight.submit(mm, kind="gradient", band_mask=np.ones(512, dtype=bool))
+        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):  # beyond the step capacity
+            tight.submit(mm, kind="gradient", band_mask=np.ones(512, dtype=bool), length_m=100.0)
         big = smf_problem(nt=32768, window_ps=100.0, length_m=0.1, power_w=0.1)
         with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
             tight.submit(big)

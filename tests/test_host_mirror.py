@@ -113,7 +113,7 @@ def test_struct_layout_matches_header():
     assert _lib.ProblemDesc.rtol.offset == 88 and _lib.ProblemDesc.max_steps.offset == 104
     assert _lib.ProblemDesc.length_m.offset == 24
     assert ctypes.sizeof(_lib.Stats) == 32
-    assert ctypes.sizeof(_lib.ServerConfig) == 8 + 4 * 4 + 3 * 8
+    assert ctypes.sizeof(_lib.ServerConfig) == 8 + 4 * 4 + 3 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.Request) == 4 * 4 + 8 + 4 * 8 + 8 + 8 + 4 * 8
     assert ctypes.sizeof(_lib.RequestResult) == 6 * 8 + 2 * 4
 

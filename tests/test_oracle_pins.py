@@ -456,3 +456,26 @@ def test_tsit5_discrete_adjoint_matches_fd_and_is_checkpoint_invariant():
         fd = (jp - jm) / (2 * eps)
         ad = 2 * np.real(np.vdot(lam, d))
         assert abs(fd - ad) <= 1e-6 * abs(ad) + 1e-12, (fd, ad)
+
+
+def test_multimode_discrete_adjoint_shared_phase_vs_fd():
+    """M = 3: band energy summed over modes (mmf_cost_sum, mmf_cost.jl:49-62),
+    shared spectral phase on all modes (mmf_raman_optimization.jl:268-271):
+    the discrete adjoint's gradient against central differences."""
+    sim, D, rf, S, u0, rng = _mm_setup(3, nt=128, seed=5)
+    g = _sym_gamma(3, rng, 5e-2)
+    fib = oracle.Fiber(D, g, rf["hR"], S, rf["one_m_fR"])
+    L, steps = 0.05, 40
+    mask = R.frequency_band_mask(sim, -16.0, -4.0)
+    J, lamL = R.band_cost(oracle.rk4_forward(fib, L, steps, u0), mask)
+    lam0 = oracle.rk4_adjoint(fib, L, steps, u0, lamL, every=7)
+    grad = np.sum(2 * np.real(np.conj(lam0) * 1j * u0), axis=1)
+    for k in (3, 40, 100):
+        e = 1e-5
+        ph = np.zeros(128)
+        ph[k] = e
+        jp = R.band_cost(oracle.rk4_forward(fib, L, steps, u0 * np.exp(1j * ph)[:, None]), mask)[0]
+        jm = R.band_cost(oracle.rk4_forward(fib, L, steps, u0 * np.exp(-1j * ph)[:, None]), mask)[0]
+        fd = (jp - jm) / (2 * e)
+        # absolute floor: FD roundoff on components far below the largest one
+        assert abs(fd - grad[k]) <= 1e-6 * abs(fd) + 1e-8 * np.abs(grad).max(), (k, fd, grad[k])
