@@ -342,8 +342,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "serving"], default="all")
     ap.add_argument("--sweep-candidates", type=int, default=256, help="adaptive candidate sweep size")
-    ap.add_argument("--serve-requests", type=int, default=300, help="requests in the serving trace")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-requests", type=int, default=100, help="requests in the serving trace")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")

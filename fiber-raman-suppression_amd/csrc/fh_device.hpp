@@ -38,33 +38,43 @@ template <int DIR, typename V> __device__ __forceinline__ V mul_i(V a)
 }
 
 // cos(2 pi m / 16) and sin(2 pi m / 16), m = 0..15
-__device__ constexpr double kCos16[16] = {
-    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
-    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
-    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
-    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
-__device__ constexpr double kSin16[16] = {
-    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
-    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
-    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
-    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+// cos and sin of 2 pi m / 32 (radix <= 32 butterfly constants)
+__device__ constexpr double kCos32[32] = {
+    1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+    0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173, 0.19509032201612826785,
+    0.0, -0.19509032201612826785, -0.38268343236508977173, -0.55557023301960222474,
+    -0.70710678118654752440, -0.83146961230254523708, -0.92387953251128675613, -0.98078528040323044913,
+    -1.0, -0.98078528040323044913, -0.92387953251128675613, -0.83146961230254523708,
+    -0.70710678118654752440, -0.55557023301960222474, -0.38268343236508977173, -0.19509032201612826785,
+    0.0, 0.19509032201612826785, 0.38268343236508977173, 0.55557023301960222474,
+    0.70710678118654752440, 0.83146961230254523708, 0.92387953251128675613, 0.98078528040323044913};
+__device__ constexpr double kSin32[32] = {
+    0.0, 0.19509032201612826785, 0.38268343236508977173, 0.55557023301960222474,
+    0.70710678118654752440, 0.83146961230254523708, 0.92387953251128675613, 0.98078528040323044913,
+    1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+    0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173, 0.19509032201612826785,
+    0.0, -0.19509032201612826785, -0.38268343236508977173, -0.55557023301960222474,
+    -0.70710678118654752440, -0.83146961230254523708, -0.92387953251128675613, -0.98078528040323044913,
+    -1.0, -0.98078528040323044913, -0.92387953251128675613, -0.83146961230254523708,
+    -0.70710678118654752440, -0.55557023301960222474, -0.38268343236508977173, -0.19509032201612826785};
 
 // multiply by w_R^m = exp(DIR * 2 pi i m / R), R | 16
 template <int R, int DIR, int M, typename V> __device__ __forceinline__ V twiddle_const(V a)
 {
-    constexpr int m = (M % R) * (16 / R);
+    static_assert(R <= 32 && 32 % R == 0, "butterfly constants cover radices up to 32");
+    constexpr int m = (M % R) * (32 / R);
     if constexpr (m == 0) {
         return a;
-    } else if constexpr (m == 8) {
+    } else if constexpr (m == 16) {
         return V{-a.x, -a.y};
-    } else if constexpr (m == 4) {
+    } else if constexpr (m == 8) {
         return mul_i<DIR>(a);
-    } else if constexpr (m == 12) {
+    } else if constexpr (m == 24) {
         return mul_i<-DIR>(a);
     } else {
         using Rt = decltype(a.x);
-        const Rt c = (Rt)kCos16[m];
-        const Rt s = (Rt)(DIR < 0 ? -kSin16[m] : kSin16[m]);
+        const Rt c = (Rt)kCos32[m];
+        const Rt s = (Rt)(DIR < 0 ? -kSin32[m] : kSin32[m]);
         return V{a.x * c - a.y * s, a.x * s + a.y * c};
     }
 }
@@ -135,6 +145,7 @@ template <int R1, int R2, int DIR, int S, typename V> struct DftSplit {
 
 template <int DIR, int S, typename V> struct Dft<8, DIR, S, V> : DftSplit<2, 4, DIR, S, V> {};
 template <int DIR, int S, typename V> struct Dft<16, DIR, S, V> : DftSplit<4, 4, DIR, S, V> {};
+template <int DIR, int S, typename V> struct Dft<32, DIR, S, V> : DftSplit<2, 16, DIR, S, V> {};
 
 // ---------------------------------------------------------------- plans
 
@@ -159,7 +170,7 @@ template <int N, int T> struct FftPlan {
     }
     static constexpr int TW_SIZE = tw_offset(NPASS);
     static_assert((1 << LOGN) == N && (1 << LOGP) == P, "power-of-two sizes only");
-    static_assert(P >= 2 && P <= 16, "2..16 points per thread");
+    static_assert(P >= 2 && P <= 32, "2..32 points per thread");
     static_assert(LOG_R0 >= 1, "radix schedule");
 };
 

@@ -11,11 +11,11 @@ template <typename R> struct MmArgs;
 // in registers; FP32 uses 16 from 4096 samples up.
 template <typename R, int N> struct SingleShape {
     static constexpr int T = sizeof(R) == 8 ? (N >= 8192 ? 1024 : (N >= 256 ? N / 8 : N / 4))
-                                            : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4));
+                                            : (N >= 32768 ? 1024 : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4)));
 };
 
 template <typename R> struct TsArgs;
-bool tsit5_supported(int nt, int m);
+bool tsit5_supported(int nt, int m, bool fp64);
 int tsit5_fields(int nt, bool fp64);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hipStream_t s);

@@ -46,6 +46,9 @@ template <typename R> hipError_t by_size(bool grad, int nt, int m, const fh::MmA
     case 16384:
         if constexpr (sizeof(R) == 4) return by_modes<R, 16384>(grad, m, a, batch, s);
         return hipErrorInvalidValue;
+    case 32768:
+        if constexpr (sizeof(R) == 4) return by_modes<R, 32768>(grad, m, a, batch, s);
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }
@@ -59,7 +62,7 @@ bool multi_supported(int nt, int m, bool fp64)
     // FP64 with two modes is not offered: its kernel returned differing bits
     // for 1-2 of 16 identical solves on some launches (N = 4096 and 8192).
     const bool mok = (m == 2 && !fp64) || m == 3 || m == 4 || m == 6 || m == 10;
-    const bool nok = nt >= 256 && nt <= (fp64 ? 8192 : 16384);
+    const bool nok = nt >= 256 && nt <= (fp64 ? 8192 : 32768);
     return mok && nok;
 }
 

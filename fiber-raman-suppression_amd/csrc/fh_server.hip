@@ -383,7 +383,7 @@ int validate(const fh_server_s* S, const fh_request* r, double* dmax)
             return fhi::set_error(FH_EUNSUPPORTED, "multimode gradient needs more steps than the server's capacity");
     }
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;
-    if (r->modes == 1 ? !fh::tsit5_supported(r->nt, 1) : !fh::multi_supported(r->nt, r->modes, fp64))
+    if (r->modes == 1 ? !fh::tsit5_supported(r->nt, 1, fp64) : !fh::multi_supported(r->nt, r->modes, fp64))
         return fhi::set_error(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(r->nt) +
                                                    " with " + std::to_string(r->modes) + " modes");
     const size_t nf = (size_t)r->nt * r->modes;

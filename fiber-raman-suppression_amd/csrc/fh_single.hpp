@@ -67,8 +67,8 @@ template <int N, int T> struct SmTraits<double, N, T> {
 };
 template <int N, int T> struct SmTraits<float, N, T> {
     static constexpr bool STATE_REGS = T <= 512;
-    static constexpr bool UT_REGS = true;
-    static constexpr bool KEEP_PHASE = true;
+    static constexpr bool UT_REGS = N / T <= 16;     // 32 samples per thread keep u in global scratch
+    static constexpr bool KEEP_PHASE = N / T <= 16;
     static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 8 > LdsBudget<float>::bytes;
 };
 

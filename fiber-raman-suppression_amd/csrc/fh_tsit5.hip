@@ -1,5 +1,5 @@
 // fh_tsit5.hip -- instantiations of the adaptive Tsit5 kernels (fh_tsit5.hpp)
-// for single-mode grids of 256..16384 samples.
+// for single-mode grids of 256..16384 samples (FP32 also 32768).
 #include <hip/hip_runtime.h>
 
 #include "fh_launch.hpp"
@@ -27,6 +27,9 @@ template <typename R> hipError_t by_size(int nt, bool grad, const fh::TsArgs<R>&
     case 4096: return go<R, 4096>(grad, a, batch, s);
     case 8192: return go<R, 8192>(grad, a, batch, s);
     case 16384: return go<R, 16384>(grad, a, batch, s);
+    case 32768:
+        if constexpr (sizeof(R) == 4) return go<R, 32768>(grad, a, batch, s);
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }
@@ -43,6 +46,9 @@ template <typename R> int fields_by_size(int nt)
     case 4096: return fields<R, 4096>();
     case 8192: return fields<R, 8192>();
     case 16384: return fields<R, 16384>();
+    case 32768:
+        if constexpr (sizeof(R) == 4) return fields<R, 32768>();
+        return 0;
     default: return 0;
     }
 }
@@ -51,7 +57,10 @@ template <typename R> int fields_by_size(int nt)
 
 namespace fh {
 
-bool tsit5_supported(int nt, int m) { return m == 1 && nt >= 256 && nt <= 16384 && (nt & (nt - 1)) == 0; }
+bool tsit5_supported(int nt, int m, bool fp64)
+{
+    return m == 1 && nt >= 256 && nt <= (fp64 ? 16384 : 32768) && (nt & (nt - 1)) == 0;
+}
 
 int tsit5_fields(int nt, bool fp64) { return fp64 ? fields_by_size<double>(nt) : fields_by_size<float>(nt); }
 

@@ -146,6 +146,9 @@ auto with_size(int nt, A&&... args) -> decltype(Op<R, 1024>::run(args...))
     case 4096: return Op<R, 4096>::run(args...);
     case 8192: return Op<R, 8192>::run(args...);
     case 16384: return Op<R, 16384>::run(args...);
+    case 32768:
+        if constexpr (sizeof(R) == 4) return Op<R, 32768>::run(args...);  // FP32 only: LDS budget
+        return Op<R, 1024>::unsupported();
     default: return Op<R, 1024>::unsupported();
     }
 }
@@ -214,8 +217,9 @@ int check_desc(const fh_problem_desc* d)
         if (!(std::isfinite(d->rtol) && d->rtol > 0)) return fail(FH_EINVAL, "rtol must be positive and finite");
         if (!(std::isfinite(d->atol) && d->atol >= 0)) return fail(FH_EINVAL, "atol must be nonnegative and finite");
         if (d->max_steps < 0) return fail(FH_EINVAL, "max_steps must be >= 0");
-        if (!fh::tsit5_supported(d->nt, d->modes))
-            return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..16384 samples");
+        if (!fh::tsit5_supported(d->nt, d->modes, d->precision == FH_PREC_FP64))
+            return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..16384 samples "
+                                         "(FP32: 256..32768)");
     }
     if (d->max_batch <= 0) return fail(FH_EINVAL, "max_batch must be positive");
     if (!(std::isfinite(d->length_m) && d->length_m > 0)) return fail(FH_EINVAL, "fiber length must be positive and finite");
@@ -241,10 +245,12 @@ int check_desc(const fh_problem_desc* d)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->modes == 1) {
-        if (d->nt < 64 || d->nt > 16384) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 16384]");
+        const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
+        if (d->nt < 64 || d->nt > nmax)
+            return fail(FH_EUNSUPPORTED, "nt must lie in [64, " + std::to_string(nmax) + "] for this precision");
     } else if (!fh::multi_supported(d->nt, d->modes, d->precision == FH_PREC_FP64)) {
         return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {3,4,6,10} with nt in [256, 8192] (FP64) and "
-                                     "M in {2,3,4,6,10} with nt in [256, 16384] (FP32)");
+                                     "M in {2,3,4,6,10} with nt in [256, 32768] (FP32)");
     }
     return FH_OK;
 }

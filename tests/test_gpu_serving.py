@@ -120,9 +120,9 @@ def test_memory_budget_admission_and_rejections(gpu_context):
         mm = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.01, energy_j=1e-9, nx=96, window_um=60.0)
         with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):  # beyond the step capacity
             tight.submit(mm, kind="gradient", band_mask=np.ones(512, dtype=bool), length_m=100.0)
-        big = smf_problem(nt=32768, window_ps=100.0, length_m=0.1, power_w=0.1)
-        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
-            tight.submit(big)
+        five = grin.grin_problem(5, nt=512, time_window_ps=4.0, length_m=0.01, energy_j=1e-9, nx=96, window_um=60.0)
+        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):  # no kernel for M = 5
+            tight.submit(five)
     with fiberhip.Server(lanes=1, max_batch=4, precision="fp32", memory_budget_bytes=1 << 20) as starved:
         with pytest.raises(fiberhip.FiberHIPError, match="FH_ENOMEM"):
             starved.wait(_submit(starved, reqs[1]))
