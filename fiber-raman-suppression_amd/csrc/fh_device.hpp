@@ -350,10 +350,27 @@ template <typename R, int N, int T, int DIR, int PASS, bool PLANAR> struct Stock
                         if (r + 1 < RAD) w = cmul(w, w1);
                     }
                 } else {
+                    // FP32: with r = Q*qa + qb, w^r = w^(Q qa) w^qb.  Loading
+                    // w^1..w^(Q-1) and w^Q, w^2Q, ... (6 loads for radix 16
+                    // instead of 15) adds one rounding to the mixed exponents;
+                    // vector-memory issue, not arithmetic, bounds these passes.
+                    constexpr int Q = RAD >= 16 ? 4 : RAD;
+                    constexpr int A = RAD / Q;
+                    V wb[Q], wa[A];
+#pragma unroll
+                    for (int q = 1; q < Q; ++q) {
+                        wb[q] = tw.ld<V>(vo, (uint32_t)((PL::tw_offset(PASS) + (q - 1) * NS) * sizeof(V)));
+                        if constexpr (DIR > 0) wb[q] = cconj(wb[q]);
+                    }
+#pragma unroll
+                    for (int q = 1; q < A; ++q) {
+                        wa[q] = tw.ld<V>(vo, (uint32_t)((PL::tw_offset(PASS) + (Q * q - 1) * NS) * sizeof(V)));
+                        if constexpr (DIR > 0) wa[q] = cconj(wa[q]);
+                    }
 #pragma unroll
                     for (int r = 1; r < RAD; ++r) {
-                        V w = tw.ld<V>(vo, (uint32_t)((PL::tw_offset(PASS) + (r - 1) * NS) * sizeof(V)));
-                        if constexpr (DIR > 0) w = cconj(w);
+                        const int qa = r / Q, qb = r % Q;
+                        const V w = qa == 0 ? wb[qb] : (qb == 0 ? wa[qa] : cmul(wa[qa], wb[qb]));
                         v[b + r * B] = cmul(v[b + r * B], w);
                     }
                 }
