@@ -12,8 +12,10 @@ template <typename R, int N> hipError_t go(bool grad, const fh::TsArgs<R>& a, in
     constexpr int T = fh::SingleShape<R, N>::T;
     if (grad)
         hipLaunchKernelGGL((fh::k_grad_tsit5<R, N, T>), dim3(batch), dim3(T), 0, s, a);
+    else if (a.nz > 0)
+        hipLaunchKernelGGL((fh::k_tsit5_single<R, N, T, true>), dim3(batch), dim3(T), 0, s, a);
     else
-        hipLaunchKernelGGL((fh::k_tsit5_single<R, N, T>), dim3(batch), dim3(T), 0, s, a);
+        hipLaunchKernelGGL((fh::k_tsit5_single<R, N, T, false>), dim3(batch), dim3(T), 0, s, a);
     return hipGetLastError();
 }
 

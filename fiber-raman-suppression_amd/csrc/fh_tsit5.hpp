@@ -38,6 +38,20 @@ struct Tsit5Tab {
                                      0.015151515151515152};
     static constexpr double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, safety = 0.9, qmin = 0.2, qmax = 10.0,
                             qoldinit = 1e-4;
+    // Continuous extension (Tsitouras 2011; the interpolant OrdinaryDiffEq
+    // evaluates for solution(t) inside a step): y(z + th h) = y + h sum_i
+    // b_i(th) k_i with b_i(th) = th (r_i0 + th (r_i1 + th (r_i2 + th r_i3))).
+    static constexpr double r[7][4] = {{1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+                                       {0.0, 0.13169999999999998, -0.2234, 0.1017},
+                                       {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+                                       {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+                                       {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+                                       {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+                                       {0.0, 1.5, -4.0, 2.5}};
+    static __device__ __forceinline__ double dense(int i, double th)
+    {
+        return th * (r[i][0] + th * (r[i][1] + th * (r[i][2] + th * r[i][3])));
+    }
 };
 
 template <typename R> struct TsArgs {
@@ -48,8 +62,52 @@ template <typename R> struct TsArgs {
     double* zs;        // [batch][max_steps] position before each accepted step
     long long* stats;  // [batch][3] accepted, rejected, rhs evaluations
     const double* lengths;  // [batch] fiber length per solve, or null for s.length
+    // dense output of the forward solve (propagate(problem; saveat))
+    const double* zsave;  // [nz] sorted save positions in [0, L], shared by the batch
+    int nz;               // 0: no dense output
+    C2<R>* saved;         // [batch][nz][N] lab-frame spectra cis(D z) y(z)
     __device__ __forceinline__ double length(size_t sol) const { return lengths ? lengths[sol] : s.length; }
 };
+
+// Lab-frame sample at save position zj of the accepted step [z, zend]: the
+// stored state at the step's ends (ode_interpolation returns stored states
+// at step boundaries) and the continuous extension inside it.
+template <typename R, int N, int T, typename KS>
+__device__ __forceinline__ void ts_save(const SmTables<R, N>& tb, const Fld& dst, double zj, double z, double h,
+                                        double zend, const Fld& y, const Fld& yn, const Fld& slot0, KS kslot)
+{
+    using V = C2<R>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = N / T;
+    const uint32_t vo = CV::voff(), FB = (uint32_t)(N * sizeof(V));
+    const bool at_start = zj == z, inside = zj != z && zj != zend;
+    R bw[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) bw[i] = inside ? (R)Tsit5Tab::dense(i, (zj - z) / h) : (R)0;
+    const R hr = (R)h;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        V val;
+        if (at_start) {
+            val = y.template ld<V>(vo, CV::soff(k));
+        } else if (!inside) {
+            val = yn.template ld<V>(vo, CV::soff(k));
+        } else {
+            V acc = V{(R)0, (R)0};
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                const V ki = Fld{slot0.b, slot0.off + (uint32_t)kslot(i) * FB}.template ld<V>(vo, CV::soff(k));
+                acc = V{acc.x + bw[i] * ki.x, acc.y + bw[i] * ki.y};
+            }
+            const V u = y.template ld<V>(vo, CV::soff(k));
+            val = V{u.x + hr * acc.x, u.y + hr * acc.y};
+        }
+        R c, s;
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), zj, c, s);
+        dst.st(V{val.x * c - val.y * s, val.x * s + val.y * c}, vo, CV::soff(k));
+    }
+}
 
 // Slot map of the workgroup slice.  Forward: y, ynew, k0..k6.  Gradient adds
 // lam, lacc; the backward sweep reuses k1..k5 for the stage cotangents.
@@ -77,7 +135,7 @@ __device__ __forceinline__ double scaled_sq(V e, V u, V w, double s, double rtol
 // or max_steps rejected steps).
 // With CKPT, the state and position at accepted steps 0, every, 2 every, ...
 // are stored to ckpt.
-template <typename R, int N, int T, bool CKPT>
+template <typename R, int N, int T, bool CKPT, bool SAVE = false>
 __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fld& slot0,
                           typename SmCtx<R, N, T>::FU& ut, const Buf& ckpt, int* ys_out, long long* nacc_out,
                           long long* nrej_out, long long* nfe_out, void* lds)
@@ -154,6 +212,19 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     double z = 0.0, qold = TB::qoldinit;
     int status = 0, last_ck = -1;
     bool done = false;
+    // Dense output (SAVE): samples at a.zsave[js..] as the accepted steps
+    // pass them, through one fixed descriptor over this solve's saved region.
+    // Kept out of the plain forward instantiation, whose arithmetic then
+    // stays identical to the gradient kernel's forward sweep.
+    int js = 0;
+    if constexpr (SAVE) {
+        const Buf sv(a.saved + sol * (size_t)a.nz * N, (uint32_t)((size_t)a.nz * FB));
+        drain_memory();
+        while (js < a.nz && a.zsave[js] == 0.0) {
+            ts_save<R, N, T>(tb, Fld{sv, (uint32_t)js * FB}, 0.0, 0.0, 0.0, 0.0, fld(sy), fld(sy), slot0, kslot);
+            ++js;
+        }
+    }
     while (!done) {
         if (nacc >= a.max_steps || nrej >= a.max_steps) {  // bounded for every wave
             status = 2;
@@ -232,6 +303,15 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
                 a.hs[sol * a.max_steps + nacc] = h;
                 a.zs[sol * a.max_steps + nacc] = z;
             }
+            if constexpr (SAVE) {
+                const Buf sv(a.saved + sol * (size_t)a.nz * N, (uint32_t)((size_t)a.nz * FB));
+                const double zend = last ? L : z + h;
+                while (js < a.nz && a.zsave[js] <= zend) {
+                    ts_save<R, N, T>(tb, Fld{sv, (uint32_t)js * FB}, a.zsave[js], z, h, zend, fld(sy), fld(syn),
+                                     slot0, kslot);
+                    ++js;
+                }
+            }
             nacc++;
             z += h;
             const int t0 = sy;
@@ -254,7 +334,7 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     return status;
 }
 
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool SAVE>
 __global__ __launch_bounds__(T) void k_tsit5_single(TsArgs<R> a)
 {
     using CX = SmCtx<R, N, T>;
@@ -281,7 +361,7 @@ __global__ __launch_bounds__(T) void k_tsit5_single(TsArgs<R> a)
     int ys;
     long long nacc, nrej, nfe;
     const Buf none(a.s.u0, 0);
-    const int status = ts_forward<R, N, T, false>(a, tb, base, ut, none, &ys, &nacc, &nrej, &nfe, lds);
+    const int status = ts_forward<R, N, T, false, SAVE>(a, tb, base, ut, none, &ys, &nacc, &nrej, &nfe, lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;

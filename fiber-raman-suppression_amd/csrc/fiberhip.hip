@@ -533,7 +533,8 @@ int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hip
 
 template <typename R>
 int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s,
-                   const double* d_lengths = nullptr, const int* d_nsteps = nullptr)
+                   const double* d_lengths = nullptr, const int* d_nsteps = nullptr, const double* d_zsave = nullptr,
+                   int nz = 0, void* d_saved = nullptr)
 {
     const fh_problem_desc& d = p->d;
     if (d.modes > 1) return launch_forward_multi<R>(p, batch, d_u0, d_uL, s, d_lengths, d_nsteps);
@@ -544,6 +545,9 @@ int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
     fh::TsArgs<R> ta = ts_args<R>(p, a);
     ta.lengths = d_lengths;
+    ta.zsave = d_zsave;
+    ta.nz = nz;
+    ta.saved = (fh::C2<R>*)d_saved;
     hipError_t e = p->adaptive ? fh::launch_tsit5(d.nt, false, ta, batch, s)
                                : with_size<R, OpLaunch>(d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -769,6 +773,132 @@ int fh_forward(fh_plan p, int32_t batch, const double* u0, double* uL, fh_stats*
         for (size_t i = 0; i < 2 * n; ++i) uL[i] = tmp[i];
     }
     return finish_call(p, batch, status, stats, false);
+}
+
+namespace {
+
+// Device memory owned by one call.
+struct TmpBuf {
+    void* p = nullptr;
+    ~TmpBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Host complex128 -> device field of the plan's precision (count complex entries).
+int upload_field(bool fp64, void* dst, const double* src, size_t count, hipStream_t s)
+{
+    if (fp64) {
+        FH_HIP(hipMemcpyAsync(dst, src, sizeof(double) * 2 * count, hipMemcpyHostToDevice, s));
+        FH_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<float> tmp(2 * count);
+        for (size_t i = 0; i < 2 * count; ++i) tmp[i] = (float)src[i];
+        FH_HIP(hipMemcpyAsync(dst, tmp.data(), sizeof(float) * 2 * count, hipMemcpyHostToDevice, s));
+        FH_HIP(hipStreamSynchronize(s));
+    }
+    return FH_OK;
+}
+
+int download_field(bool fp64, double* dst, const void* src, size_t count, hipStream_t s)
+{
+    if (fp64) {
+        FH_HIP(hipMemcpyAsync(dst, src, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+    } else {
+        std::vector<float> tmp(2 * count);
+        FH_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(float) * 2 * count, hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < 2 * count; ++i) dst[i] = tmp[i];
+    }
+    return FH_OK;
+}
+
+}  // namespace
+
+int fh_forward_saveat(fh_plan p, int32_t batch, const double* u0, double* uL, int32_t nz, const double* zsave,
+                      double* saved, fh_stats* stats)
+{
+    if (!p || !u0 || !uL || !zsave || !saved) return fail(FH_EINVAL, "plan, field or save pointer is NULL");
+    if (!p->adaptive || p->d.modes != 1)
+        return fail(FH_EUNSUPPORTED, "dense output (saveat) needs an adaptive (FH_SCHEME_TSIT5) single-mode plan");
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    const size_t nt = (size_t)p->d.nt;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    const size_t esz = fp64 ? 16 : 8;
+    if (nz < 1 || (size_t)nz * nt * esz >= ((size_t)1 << 31))
+        return fail(FH_EINVAL, "nz must lie in [1, 2^31 / (nt * element size))");
+    for (int j = 0; j < nz; ++j) {
+        if (!std::isfinite(zsave[j])) return fail(FH_EINVAL, "saveat distances must be finite");
+        if (zsave[j] < 0 || zsave[j] > p->d.length_m)
+            return fail(FH_EINVAL, "saveat distances must lie in [0, L]");
+        if (j > 0 && !(zsave[j] > zsave[j - 1])) return fail(FH_EINVAL, "saveat distances must be sorted and unique");
+    }
+    const size_t n = (size_t)batch * nt;
+    double norm2 = 0.0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");
+        norm2 += u0[i] * u0[i];
+    }
+    if (!(norm2 > 0)) return fail(FH_EINVAL, "input spectrum must be nonzero");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    TmpBuf dz, dsv;
+    FH_HIP(hipMalloc(&dz.p, sizeof(double) * (size_t)nz));
+    FH_HIP(hipMalloc(&dsv.p, esz * (size_t)nz * n));
+    FH_HIP(hipMemcpyAsync(dz.p, zsave, sizeof(double) * (size_t)nz, hipMemcpyHostToDevice, s));
+    int rc = upload_field(fp64, p->u_in.p, u0, n, s);
+    if (rc) return rc;
+    rc = fp64 ? launch_forward<double>(p, batch, p->u_in.p, p->u_out.p, s, nullptr, nullptr, (const double*)dz.p,
+                                       nz, dsv.p)
+              : launch_forward<float>(p, batch, p->u_in.p, p->u_out.p, s, nullptr, nullptr, (const double*)dz.p, nz,
+                                      dsv.p);
+    if (rc) return rc;
+    std::vector<int> status(batch);
+    FH_HIP(hipMemcpyAsync(status.data(), p->status.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    if ((rc = download_field(fp64, uL, p->u_out.p, n, s))) return rc;
+    if ((rc = download_field(fp64, saved, dsv.p, (size_t)nz * n, s))) return rc;
+    return finish_call(p, batch, status, stats, false);
+}
+
+int fh_field_metrics(fh_plan p, int32_t count, const double* spectra, const double* omega, double dt_ps, double* out)
+{
+    if (!p || !spectra || !omega || !out) return fail(FH_EINVAL, "plan, spectra, omega or out pointer is NULL");
+    if (count < 1) return fail(FH_EINVAL, "count must be >= 1");
+    if (!(std::isfinite(dt_ps) && dt_ps > 0)) return fail(FH_EINVAL, "dt must be positive and finite");
+    const size_t nt = (size_t)p->d.nt, m = (size_t)p->d.modes;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    const size_t esz = fp64 ? 16 : 8;
+    if (m * nt * esz >= ((size_t)1 << 31)) return fail(FH_EINVAL, "spectrum too large");
+    for (size_t k = 0; k < nt; ++k)
+        if (!(std::isfinite(omega[k]) && omega[k] > 0)) return fail(FH_EINVAL, "omega must be positive and finite");
+    const size_t per = m * nt;
+    for (int c = 0; c < count; ++c) {
+        double norm2 = 0.0;
+        for (size_t i = 0; i < 2 * per; ++i) {
+            const double x = spectra[(size_t)c * 2 * per + i];
+            if (!std::isfinite(x)) return fail(FH_EINVAL, "spectra must be finite");
+            norm2 += x * x;
+        }
+        if (!(norm2 > 0)) return fail(FH_EINVAL, "field power must be nonzero");
+    }
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    TmpBuf dsp, dom, dout;
+    FH_HIP(hipMalloc(&dsp.p, esz * per * (size_t)count));
+    FH_HIP(hipMalloc(&dom.p, sizeof(double) * nt));
+    FH_HIP(hipMalloc(&dout.p, sizeof(double) * (m + 5) * (size_t)count));
+    FH_HIP(hipMemcpyAsync(dom.p, omega, sizeof(double) * nt, hipMemcpyHostToDevice, s));
+    int rc = upload_field(fp64, dsp.p, spectra, per * (size_t)count, s);
+    if (rc) return rc;
+    hipError_t e = fh::launch_field_metrics((int)nt, fp64, dsp.p, p->tw.p, (const double*)dom.p, dt_ps, (int)m,
+                                            (double*)dout.p, count, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("metrics launch: ") + hipGetErrorString(e));
+    FH_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * (m + 5) * (size_t)count, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    return FH_OK;
 }
 
 int fh_device_alloc(fh_context ctx, int64_t bytes, void** ptr)

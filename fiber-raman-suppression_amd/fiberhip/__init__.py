@@ -44,7 +44,6 @@ from .adjoint import (  # noqa: F401
 from .propagation import (  # noqa: F401
     PropagationResult,
     centroid_thz,
-    field_metrics,
     metrics,
     propagate,
     scalar_rk4_output,

@@ -144,6 +144,9 @@ _SIGNATURES = {
     "fh_plan_destroy": ([_VP], ctypes.c_int),
     "fh_plan_device_bytes": ([_VP, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "fh_forward_saveat": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)],
+                          ctypes.c_int),
+    "fh_field_metrics": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_double, _DP], ctypes.c_int),
     "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
     "fh_gradient": ([_VP, ctypes.c_int32, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),

@@ -194,6 +194,48 @@ class Plan:
                      retcode=st.retcode)
         return (res[0] if single else res), stats
 
+    def forward_saveat(self, u0, zsave) -> tuple[np.ndarray, dict, np.ndarray]:
+        """Forward solve with dense output (fh_forward_saveat): lab-frame spectra
+        at the sorted positions zsave, shape (nz, nt, m) per launch, beside the
+        output and stats of forward()."""
+        u = np.asarray(u0, dtype=np.complex128)
+        single = u.ndim == 2
+        if single:
+            u = u[None]
+        if u.ndim != 3 or u.shape[1:] != (self.nt, self.modes):
+            raise ValueError(f"launch has shape {u.shape}, expected (batch, {self.nt}, {self.modes})")
+        z = np.ascontiguousarray(zsave, dtype=np.float64).ravel()
+        batch = u.shape[0]
+        buf = np.ascontiguousarray(np.transpose(u, (0, 2, 1)))  # [batch][mode][t]
+        out = np.empty_like(buf)
+        saved = np.empty((batch, z.size, self.modes, self.nt), dtype=np.complex128)
+        st = _lib.Stats()
+        check(lib().fh_forward_saveat(self.handle, batch, _dp(buf.view(np.float64)), _dp(out.view(np.float64)),
+                                      int(z.size), _dp(z), _dp(saved.view(np.float64)), ctypes.byref(st)))
+        res = np.transpose(out, (0, 2, 1))
+        sv = np.transpose(saved, (0, 1, 3, 2))
+        stats = dict(accepted=st.accepted, rejected=st.rejected, rhs_evaluations=st.rhs_evals,
+                     retcode=st.retcode)
+        return (res[0] if single else res), stats, (sv[0] if single else sv)
+
+    def field_metrics(self, spectra, sim: dict) -> list:
+        """FP64 evidence of lab-frame spectra (fh_field_metrics; _field_metrics,
+        propagation.jl:184-202).  spectra: (nt, m) or (count, nt, m)."""
+        sp = np.asarray(spectra, dtype=np.complex128)
+        if sp.ndim == 2:
+            sp = sp[None]
+        if sp.ndim != 3 or sp.shape[1:] != (self.nt, self.modes):
+            raise ValueError(f"spectra have shape {sp.shape}, expected (count, {self.nt}, {self.modes})")
+        buf = np.ascontiguousarray(np.transpose(sp, (0, 2, 1)))
+        omega = np.ascontiguousarray(np.fft.fftshift(np.abs(np.asarray(sim["ωs"], dtype=np.float64))))
+        out = np.zeros((sp.shape[0], self.modes + 5))
+        check(lib().fh_field_metrics(self.handle, sp.shape[0], _dp(buf.view(np.float64)), _dp(omega),
+                                     float(sim["Δt"]), _dp(out)))
+        m = self.modes
+        return [dict(energy_pj=float(r[m]), mode_energy_pj=r[:m].copy(), peak_power_w=float(r[m + 1]),
+                     temporal_edge_fraction=float(r[m + 2]), spectral_edge_fraction=float(r[m + 3]),
+                     photon_number=float(r[m + 4])) for r in out]
+
     def gradient(self, u0, seed=None) -> dict:
         """Discrete-adjoint gradient (fh_gradient).  u0: (nt, m) or (batch, nt, m)
         shaped launches; seed: matching Wirtinger terminal seeds, or None for

@@ -123,6 +123,17 @@ int fh_plan_device_bytes(fh_plan plan, int64_t *bytes);
  * Host pointers; blocks until done. */
 int fh_forward(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_stats *stats);
 
+/* fh_forward plus dense output: the lab-frame spectra cis(D z) y(z) at the
+ * nz sorted, unique positions zsave (0 <= z <= L), which
+ * propagate(problem; saveat) stores (propagation.jl:89-123, samples taken by
+ * _lab_spectrum from the ODE solution, :34-36).  Positions on an accepted
+ * step boundary return the stored state; positions inside a step use the
+ * solver's 4th-order continuous extension.  saved: batch*nz*modes*nt
+ * complex128, ordered [batch][z][mode][t].  Adaptive (FH_SCHEME_TSIT5)
+ * single-mode plans only (FH_EUNSUPPORTED otherwise). */
+int fh_forward_saveat(fh_plan plan, int32_t batch, const double *u0, double *uL, int32_t nz, const double *zsave,
+                      double *saved, fh_stats *stats);
+
 /* Same solve on device-resident arrays in the plan's precision (complex128
  * for FH_PREC_FP64, complex64 for FH_PREC_FP32_BULK), enqueued on `stream`
  * (a hipStream_t; NULL = the context stream).  Does not synchronise. */
@@ -161,6 +172,17 @@ int fh_time_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, void 
  * (the mode_energy of _field_metrics, propagation.jl:189-202, up to the
  * Parseval factor nt*dt). */
 int fh_energy_device(fh_plan plan, int32_t batch, const void *d_u, double *energies);
+
+/* FP64 evidence of `count` lab-frame spectra (host, complex128,
+ * [count][mode][t], nt and modes of the plan): the per-sample quantities of
+ * _field_metrics / _photon_number / _edge_fraction (propagation.jl:176-202)
+ * that metrics() and verify() gate on.  omega: |omega_s| in FFT order
+ * (rad/ps, fftshift(abs.(sim["ws"]))); dt_ps: sim["dt"].  out[count][modes+5]:
+ * mode energies (pJ), total energy (pJ), peak power (W), temporal edge
+ * fraction, spectral edge fraction, photon number.  The time-domain power
+ * uses the plan's transform and precision; every sum is FP64. */
+int fh_field_metrics(fh_plan plan, int32_t count, const double *spectra, const double *omega, double dt_ps,
+                     double *out);
 
 /* Device buffers on the context's device (plain hipMalloc / hipMemcpy on the
  * context stream).  Callers without their own HIP allocator use these to

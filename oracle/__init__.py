@@ -71,11 +71,14 @@ def lib() -> ctypes.CDLL:
         _LIB.orc_tsit5_tableau.argtypes = [dp, dp, dp]
         _LIB.orc_tsit5_forward.argtypes = [fp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            dp, dp, dp, i64p]
+        _LIB.orc_tsit5_forward_save.argtypes = [fp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, dp, dp, dp, i64p, ctypes.c_int, dp, dp]
+        _LIB.orc_tsit5_dense_weights.argtypes = [ctypes.c_double, dp]
         _LIB.orc_tsit5_replay.argtypes = [fp, ctypes.c_double, dp, ctypes.c_int, dp, dp]
         _LIB.orc_tsit5_adjoint.argtypes = [fp, ctypes.c_double, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
         for name in ("orc_fft", "orc_rhs", "orc_rhs_vjp", "orc_rk4_forward", "orc_rk4_adjoint",
                      "orc_adjoint_rhs", "orc_tsit5_tableau", "orc_tsit5_forward", "orc_tsit5_replay",
-                     "orc_tsit5_adjoint"):
+                     "orc_tsit5_adjoint", "orc_tsit5_forward_save", "orc_tsit5_dense_weights"):
             getattr(_LIB, name).restype = ctypes.c_int
     return _LIB
 
@@ -202,9 +205,16 @@ def tsit5_tableau():
     return c, a, bt
 
 
+def tsit5_dense_weights(theta: float) -> np.ndarray:
+    """b_i(theta), i = 1..7, of the Tsit5 continuous extension."""
+    b = np.zeros(7)
+    _check(lib().orc_tsit5_dense_weights(ctypes.c_double(theta), _dp(b)), "orc_tsit5_dense_weights")
+    return b
+
+
 class Tsit5Result:
-    def __init__(self, out, hs, stats):
-        self.out, self.hs, self.stats = out, hs, stats
+    def __init__(self, out, hs, stats, saved=None):
+        self.out, self.hs, self.stats, self.saved = out, hs, stats, saved
 
     @property
     def accepted(self) -> int:
@@ -220,20 +230,26 @@ class Tsit5Result:
 
 
 def tsit5_forward(fib: Fiber, length: float, u0, rtol: float = 1e-6, atol: float = 1e-6,
-                  max_steps: int = 100000) -> Tsit5Result:
+                  max_steps: int = 100000, saveat=None) -> Tsit5Result:
     """Adaptive Tsit5 (solve_disp_mmf, simulate_disp_mmf.jl:183-207) -> lab spectrum,
-    accepted step sizes and {accepted, rejected, rhs evaluations}."""
+    accepted step sizes and {accepted, rejected, rhs evaluations}.  With
+    `saveat` (sorted positions in [0, L]) also the lab spectra there,
+    shape (nz, nt, m), from the continuous extension."""
     u0 = _field(u0)
     out = np.zeros_like(u0)
     hs = np.zeros(max_steps)
     st = np.zeros(3, dtype=np.int64)
-    rc = lib().orc_tsit5_forward(fib.ref, length, rtol, atol, max_steps, _dp(u0.view(np.float64)),
-                                 _dp(out.view(np.float64)), _dp(hs),
-                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    z = np.ascontiguousarray([] if saveat is None else saveat, dtype=np.float64)
+    saved = np.zeros((z.size,) + u0.shape, dtype=np.complex128)
+    rc = lib().orc_tsit5_forward_save(fib.ref, length, rtol, atol, max_steps, _dp(u0.view(np.float64)),
+                                      _dp(out.view(np.float64)), _dp(hs),
+                                      st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(z.size), _dp(z),
+                                      _dp(saved.view(np.float64)))
     if rc == -3:
         raise RuntimeError("orc_tsit5_forward: max_steps reached")
     _check(rc, "orc_tsit5_forward")
-    return Tsit5Result(out.T, hs[: st[0]].copy(), st)
+    return Tsit5Result(out.T, hs[: st[0]].copy(), st,
+                       None if saveat is None else np.transpose(saved, (0, 2, 1)).copy())
 
 
 def tsit5_replay(fib: Fiber, length: float, hs, u0) -> np.ndarray:
@@ -257,6 +273,6 @@ def tsit5_adjoint(fib: Fiber, length: float, hs, u0, lam_out, every: int = 50) -
 
 
 __all__ = ["Fiber", "build", "lib", "fft", "rhs", "rhs_vjp", "rk4_forward", "rk4_adjoint",
-           "adjoint_rhs", "continuous_adjoint", "tsit5_tableau", "tsit5_forward", "tsit5_replay",
+           "adjoint_rhs", "continuous_adjoint", "tsit5_tableau", "tsit5_dense_weights", "tsit5_forward", "tsit5_replay",
            "tsit5_adjoint", "np_ref"]
 _ = os  # keep import for callers that patch paths

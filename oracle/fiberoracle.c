@@ -695,6 +695,32 @@ int orc_tsit5_tableau(double *c, double *a, double *bt)
     return 0;
 }
 
+/* Continuous extension of the pair (Tsitouras 2011, the free 4th-order
+ * interpolant OrdinaryDiffEq evaluates for solution(t) between accepted
+ * steps, which propagate(problem; saveat) samples through _lab_spectrum,
+ * propagation.jl:34-36,106-108): y(z + th h) = y + h sum_i b_i(th) k_i with
+ * b_i(th) = sum_p TS_R[i][p] th^(p+1).  b_i(1) equals row 7 of TS_A. */
+static const double TS_R[7][4] = {
+    {1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+    {0.0, 0.13169999999999998, -0.2234, 0.1017},
+    {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+    {0.0, 1.5, -4.0, 2.5}};
+
+static void dense_weights(double th, double *b)
+{
+    for (int i = 0; i < 7; ++i)
+        b[i] = th * (TS_R[i][0] + th * (TS_R[i][1] + th * (TS_R[i][2] + th * TS_R[i][3])));
+}
+
+int orc_tsit5_dense_weights(double theta, double *b)
+{
+    dense_weights(theta, b);
+    return 0;
+}
+
 /* Step-size control, restating OrdinaryDiffEq's defaults for a 5th-order
  * explicit pair: error norm sqrt(mean |e|^2) with e = utilde / (atol +
  * max(|u_prev|, |u_new|) rtol) per complex entry; PI controller with
@@ -746,15 +772,47 @@ static double tsit5_attempt(const orc_fiber *f, orc_ws *ws, double z, double h, 
     return scaled_norm(tmp, y, ynew, nf, rtol, atol, fabs(h));
 }
 
+/* Lab-frame sample at a save position: cis(D zj) times the state at zj, which
+ * is y (zj = z), ynew (zj = the step's end) or the continuous extension
+ * y + h sum_i b_i(th) k_i in between (ode_interpolation returns stored
+ * states at step boundaries and interpolates inside a step). */
+static void save_sample(const orc_fiber *f, double zj, double z, double h, double zend, const cplx *y,
+                        const cplx *ynew, const cplx *k, cplx *dst)
+{
+    const size_t nf = (size_t)f->nt * f->m;
+    double b[7];
+    const int inside = zj != z && zj != zend;
+    if (inside) dense_weights((zj - z) / h, b);
+    for (size_t o = 0; o < nf; ++o) {
+        cplx val;
+        if (zj == z) {
+            val = y[o];
+        } else if (!inside) {
+            val = ynew[o];
+        } else {
+            cplx s = 0;
+            for (int i = 0; i < 7; ++i) s += b[i] * k[(size_t)i * nf + o];
+            val = y[o] + h * s;
+        }
+        dst[o] = cis(f->disp[o] * zj) * val;
+    }
+}
+
 /* Adaptive Tsit5 from z = 0 to L (solve_disp_mmf with reltol/abstol,
  * simulate_disp_mmf.jl:183-207).  out = cis(D L) y(L).  hs (may be NULL,
  * max_steps entries) receives the accepted step sizes; stats = {accepted,
- * rejected, rhs evaluations}.  Returns 0, -1 (bad args / memory), -2
- * (non-finite) or -3 (max_steps reached). */
-int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double atol, int max_steps,
-                      const double *u0, double *out, double *hs, int64_t *stats)
+ * rejected, rhs evaluations}.  With nz > 0, saved[j] (nz fields) receives
+ * the lab-frame spectrum at zsave[j] (sorted, in [0, L]), the samples
+ * propagate(problem; saveat) stores (propagation.jl:99-110).  Returns 0, -1
+ * (bad args / memory), -2 (non-finite) or -3 (max_steps reached). */
+int orc_tsit5_forward_save(const orc_fiber *f, double length_m, double rtol, double atol, int max_steps,
+                           const double *u0, double *out, double *hs, int64_t *stats, int nz,
+                           const double *zsave, double *saved)
 {
-    if (length_m <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0) return -1;
+    if (length_m <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0 || nz < 0 || (nz > 0 && (!zsave || !saved)))
+        return -1;
+    for (int j = 0; j < nz; ++j)
+        if (!(zsave[j] >= 0 && zsave[j] <= length_m) || (j > 0 && !(zsave[j] > zsave[j - 1]))) return -1;
     const size_t nf = (size_t)f->nt * f->m;
     orc_ws ws;
     if (ws_init(&ws, f->nt, f->m)) return -1;
@@ -763,7 +821,12 @@ int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double a
     if (!k || !y || !ynew || !tmp) { free(k); free(y); free(ynew); free(tmp); ws_free(&ws); return -1; }
     memcpy(y, u0, sizeof(cplx) * nf);
     int64_t nacc = 0, nrej = 0, nfe = 0;
-    int rc = 0;
+    int rc = 0, js = 0;
+    cplx *sv = (cplx *)saved;
+    while (js < nz && zsave[js] == 0.0) {
+        save_sample(f, 0.0, 0.0, 0.0, 0.0, y, y, NULL, sv + (size_t)js * nf);
+        ++js;
+    }
     /* starting step */
     rhs_eval(f, &ws, 0.0, y, k);
     nfe++;
@@ -799,6 +862,11 @@ int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double a
                 if (q > 1.0 / TS_QMIN) q = 1.0 / TS_QMIN;
                 qold = e > TS_QOLDINIT ? e : TS_QOLDINIT;
                 if (hs) hs[nacc] = h;
+                const double zend = last ? length_m : z + h;
+                while (js < nz && zsave[js] <= zend) {
+                    save_sample(f, zsave[js], z, h, zend, y, ynew, k, sv + (size_t)js * nf);
+                    ++js;
+                }
                 nacc++;
                 z += h;
                 memcpy(y, ynew, sizeof(cplx) * nf);
@@ -824,6 +892,12 @@ int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double a
     free(k); free(y); free(ynew); free(tmp);
     ws_free(&ws);
     return rc;
+}
+
+int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double atol, int max_steps,
+                      const double *u0, double *out, double *hs, int64_t *stats)
+{
+    return orc_tsit5_forward_save(f, length_m, rtol, atol, max_steps, u0, out, hs, stats, 0, NULL, NULL);
 }
 
 /* One Tsit5 step with a given h (no error control), 6 stages; b = row 7 of A.

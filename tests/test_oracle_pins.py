@@ -405,6 +405,52 @@ def test_tsit5_cw_kerr_tight_and_loose():
     assert tight < 1e-8 and tight < 1e-3 * loose
 
 
+def test_tsit5_dense_output_order_conditions():
+    """The continuous extension b_i(th) satisfies the eight order conditions
+    through order 4 at every th, vanishes at th = 0 and equals the 5th-order
+    weights at th = 1.  The coefficient table is overdetermined by these
+    (22 free coefficients, > 27 conditions), so a mistyped digit fails."""
+    c, a, _ = oracle.tsit5_tableau()
+    A = np.zeros((7, 7))
+    A[:, :6] = a
+    Ac = A @ c
+    assert np.max(np.abs(oracle.tsit5_dense_weights(0.0))) == 0.0
+    assert np.max(np.abs(oracle.tsit5_dense_weights(1.0) - A[6])) < 1e-14
+    for th in (0.05, 0.21, 0.5, 0.77, 0.93, 1.0):
+        w = oracle.tsit5_dense_weights(th)
+        conds = [w.sum() - th, w @ c - th ** 2 / 2, w @ c ** 2 - th ** 3 / 3, w @ Ac - th ** 3 / 6,
+                 w @ c ** 3 - th ** 4 / 4, w @ (c * Ac) - th ** 4 / 8, w @ (A @ c ** 2) - th ** 4 / 12,
+                 w @ (A @ Ac) - th ** 4 / 24]
+        assert np.max(np.abs(conds)) < 1e-13, (th, conds)
+
+
+def test_tsit5_saveat_closed_forms():
+    """propagate(problem; saveat) samples (propagation.jl:99-110) from the
+    adaptive solver's continuous extension.  CW Kerr (test_numerical_kernels.jl:
+    329-353 setup): positions inside accepted steps match u0 cis(gamma P z)
+    to < 1e-8 at (1e-10, 1e-12), far below the 1e-2 a sample-and-hold of the
+    previous accepted state would leave.  gamma = 0 with dispersion: the
+    samples are cis(D z) u0.  Endpoints are the stored states exactly."""
+    nt = 64
+    launch = np.fft.ifft(np.full((nt, 1), np.sqrt(2.0), dtype=complex), axis=0)
+    fib = oracle.Fiber(np.zeros((nt, 1)), np.full((1, 1, 1, 1), 0.73), np.zeros(nt), np.ones(nt), 1.0)
+    zs = np.array([0.0, 0.37, 1.234, 2.5, 3.9, 5.0])
+    r = oracle.tsit5_forward(fib, 5.0, launch, rtol=1e-10, atol=1e-12, saveat=zs)
+    bounds = np.concatenate([[0.0], np.cumsum(r.hs)])
+    inside = [z for z in zs[1:-1] if np.min(np.abs(bounds - z)) > 1e-6]
+    assert len(inside) >= 3
+    for j, z in enumerate(zs):
+        assert _rel(r.saved[j], launch * np.exp(1j * 0.73 * 2.0 * z)) < 1e-8, z
+    assert np.array_equal(r.saved[0], launch) and np.array_equal(r.saved[-1], r.out)
+
+    sim, fib_lin, u0, (D, _, hR, S, omf) = _smf(nt=128, window=4.0, length=0.3, power=0.05)
+    lin = oracle.Fiber(D, np.zeros((1, 1, 1, 1)), hR, S, omf)
+    zl = np.array([0.0, 0.0123, 0.1, 0.2999, 0.3])
+    rl = oracle.tsit5_forward(lin, 0.3, u0, rtol=1e-8, atol=1e-6, saveat=zl)
+    for j, z in enumerate(zl):
+        assert _rel(rl.saved[j], np.exp(1j * D * z) * u0) < 1e-13, z
+
+
 def test_tsit5_strict_vs_rk4_crosscheck():
     """test_scalar_crosscheck.jl:58-75: SMF28 0.02 m, 0.02 W, Nt=128, 4 ps;
     128 fixed RK4 steps vs the adaptive solver at (1e-11, 1e-12): < 1e-7,
