@@ -23,6 +23,11 @@ hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hi
 hipError_t launch_field_metrics(int nt, bool fp64, const void* spec, const void* tw, const double* omega, double dt,
                                 int modes, double* out, int count, hipStream_t s);
 
+template <typename R> struct MtArgs;
+int tsit5_multi_fields(int m);
+hipError_t launch_tsit5_multi(int nt, int m, const MtArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_tsit5_multi(int nt, int m, const MtArgs<float>& a, int batch, hipStream_t s);
+
 bool multi_supported(int nt, int m, bool fp64);
 int multi_fields(int m);
 int multi_grad_fields(int m);

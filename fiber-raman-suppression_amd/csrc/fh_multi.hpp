@@ -91,19 +91,20 @@ template <typename R, int N, int T, int M> struct MmCtx {
     __device__ __forceinline__ Fld fld(int f) const { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; }
 };
 
-// One RK4 stage s of the step from state st at position zs (steps 1-4 of the
-// right-hand side fused with the RK4 bookkeeping of y/st/ac).  With TAPE the
-// stage's time fields u_m and final h pairs are also stored to tape slot
-// `tslot` (tape_u: M fields per slot, tape_h: QC fields per slot).
+// Steps 1-3 of the right-hand side at zs: time fields u_m = fft(cis(D_m zs)
+// y_m) of the input fields y_m (slice fields IN + m) into U + m, and the
+// pair fields h (Kerr contraction plus packed Raman) into HQ + cc.  With TAPE
+// the stage's u_m and final h pairs are also stored to tape slot `tslot`
+// (tape_u: M fields per slot, tape_h: QC fields per slot).
 template <typename R, int N, int T, int M, bool TAPE>
-__device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, double zs, double h, const Buf& tape_u,
-                                         const Buf& tape_h, uint32_t tslot, C2<R>* v, void* lds)
+__device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double zs, int IN, int U, int HQ,
+                                            const Buf& tape_u, const Buf& tape_h, uint32_t tslot, C2<R>* v,
+                                            void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
     using CV = Cyc<V, T>;
     using CD = Cyc<double, T>;
-    using CR = Cyc<R, T>;
     constexpr int P = TR::P;
     constexpr int NP = TR::NP;
     const MmArgs<R>& a = c.a;
@@ -111,12 +112,11 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
     const uint32_t FB = (uint32_t)(N * sizeof(V));
     const int Q = c.Q, QC = c.QC;
     const R* __restrict__ gr = a.gr;
-    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
-    drain_memory();  // re-reads y / st / ac of the previous stage
+    drain_memory();  // re-reads the input fields written by the previous stage
     // 1. u_m = fft(cis(D_m zs) y_m)
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(m), u = c.fld(3 * M + m);
+        const Fld y = c.fld(IN + m), u = c.fld(U + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R cs, sn;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
     for (int k = 0; k < P; ++k) {
         V uk[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) uk[m] = c.fld(3 * M + m).template ld<V>(vo, CV::soff(k));
+        for (int m = 0; m < M; ++m) uk[m] = c.fld(U + m).template ld<V>(vo, CV::soff(k));
         R b[NP];
         {
             int p = 0;
@@ -159,14 +159,14 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
 #pragma unroll
                 for (int p = 0; p < NP; ++p) d1 += g1[p] * b[p];
             }
-            c.fld(4 * M + cc).st(V{d0, d1}, vo, CV::soff(k));
+            c.fld(HQ + cc).st(V{d0, d1}, vo, CV::soff(k));
         }
     }
     drain_memory();  // re-reads hq of step 2
     // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))
 #pragma unroll 1
     for (int cc = 0; cc < QC; ++cc) {
-        const Fld hq = c.fld(4 * M + cc);
+        const Fld hq = c.fld(HQ + cc);
 #pragma unroll
         for (int k = 0; k < P; ++k) v[k] = hq.template ld<V>(vo, CV::soff(k));
         block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
@@ -182,6 +182,24 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
             if constexpr (TAPE) th.st(hv, vo, CV::soff(k));
         }
     }
+}
+
+// One RK4 stage s of the step from state st at position zs (the right-hand
+// side fused with the RK4 bookkeeping of y/st/ac).  Tape as in mm_rhs_head.
+template <typename R, int N, int T, int M, bool TAPE>
+__device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, double zs, double h, const Buf& tape_u,
+                                         const Buf& tape_h, uint32_t tslot, C2<R>* v, void* lds)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    using CR = Cyc<R, T>;
+    constexpr int P = TR::P;
+    const MmArgs<R>& a = c.a;
+    const uint32_t vo = CV::voff();
+    const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
+    mm_rhs_head<R, N, T, M, TAPE>(c, zs, 0, 3 * M, 4 * M, tape_u, tape_h, tslot, v, lds);
     drain_memory();  // re-reads hq of step 3
     // 4. per mode: eta_i = sum_j h_ij u_j; k_i; RK4 bookkeeping
 #pragma unroll 1
@@ -227,6 +245,53 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
             }
             if (s != 3) ac.st(cc, vo, CV::soff(k));
             y.st(yn, vo, CV::soff(k));
+        }
+    }
+}
+
+// Full right-hand side at zs for the adaptive driver: k_i = f_i(zs, y) of the
+// input fields IN + i into the output fields OUT + i (U, HQ: scratch fields
+// of mm_rhs_head).
+template <typename R, int N, int T, int M>
+__device__ __forceinline__ void mm_rhs(const MmCtx<R, N, T, M>& c, double zs, int IN, int OUT, int U, int HQ,
+                                       C2<R>* v, void* lds)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    using CR = Cyc<R, T>;
+    constexpr int P = TR::P;
+    const MmArgs<R>& a = c.a;
+    const uint32_t vo = CV::voff();
+    const Buf none(a.u0, 0);
+    mm_rhs_head<R, N, T, M, false>(c, zs, IN, U, HQ, none, none, 0, v, lds);
+    drain_memory();  // re-reads hq of step 3
+    // 4. per mode: eta_i = sum_j h_ij u_j, k_i = i cis(-D_i zs) shock ifft(eta_i)
+#pragma unroll 1
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            V acc = V{(R)0, (R)0};
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const int q = a.qidx[i * M + j];
+                const V hp = c.fld(HQ + (q >> 1)).template ld<V>(vo, CV::soff(k));
+                const R hij = (q & 1) ? hp.y : hp.x;
+                const V uj = c.fld(U + j).template ld<V>(vo, CV::soff(k));
+                acc = V{acc.x + hij * uj.x, acc.y + hij * uj.y};
+            }
+            v[k] = acc;
+        }
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+        const Fld out = c.fld(OUT + i);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+            const V e = V{v[k].x * sc, v[k].y * sc};
+            out.st(V{sn * e.x - cs * e.y, sn * e.y + cs * e.x}, vo, CV::soff(k));
         }
     }
 }

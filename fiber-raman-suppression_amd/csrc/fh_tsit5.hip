@@ -61,6 +61,7 @@ namespace fh {
 
 bool tsit5_supported(int nt, int m, bool fp64)
 {
+    if (m > 1) return multi_supported(nt, m, fp64);
     return m == 1 && nt >= 256 && nt <= (fp64 ? 16384 : 32768) && (nt & (nt - 1)) == 0;
 }
 

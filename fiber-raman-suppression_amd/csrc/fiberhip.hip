@@ -15,6 +15,7 @@
 #include "fh_multi.hpp"
 #include "fh_single.hpp"
 #include "fh_tsit5.hpp"
+#include "fh_tsit5_multi.hpp"
 
 namespace {
 
@@ -219,7 +220,8 @@ int check_desc(const fh_problem_desc* d)
         if (d->max_steps < 0) return fail(FH_EINVAL, "max_steps must be >= 0");
         if (!fh::tsit5_supported(d->nt, d->modes, d->precision == FH_PREC_FP64))
             return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..16384 samples "
-                                         "(FP32: 256..32768)");
+                                         "(FP32: 256..32768) and the multimode grids and mode counts of the "
+                                         "fixed-step scheme");
     }
     if (d->max_batch <= 0) return fail(FH_EINVAL, "max_batch must be positive");
     if (!(std::isfinite(d->length_m) && d->length_m > 0)) return fail(FH_EINVAL, "fiber length must be positive and finite");
@@ -296,7 +298,8 @@ template <typename R> int upload_tables(fh_plan p)
         p->max_steps = d.max_steps > 0 ? d.max_steps : 10000;
         if (p->every > p->max_steps) p->every = p->max_steps;
         p->nseg = (p->max_steps + p->every - 1) / p->every;
-        p->work_per_solve = (size_t)fh::tsit5_fields(nt, sizeof(R) == 8) * nt;
+        p->work_per_solve = d.modes == 1 ? (size_t)fh::tsit5_fields(nt, sizeof(R) == 8) * nt
+                                         : (size_t)fh::tsit5_multi_fields(d.modes) * nt;
     } else {
         if (p->every > d.steps) p->every = d.steps;
         p->nseg = (d.steps + p->every - 1) / p->every;
@@ -504,7 +507,8 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
 
 template <typename R>
 int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream_t s,
-                         const double* d_lengths = nullptr, const int* d_nsteps = nullptr)
+                         const double* d_lengths = nullptr, const int* d_nsteps = nullptr,
+                         const double* d_zsave = nullptr, int nz = 0, void* d_saved = nullptr)
 {
     const fh_problem_desc& d = p->d;
     fh::MmArgs<R> a{};
@@ -526,6 +530,22 @@ int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hip
     a.lengths = d_lengths;
     a.nsteps = d_nsteps;
     FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    if (p->adaptive) {
+        fh::MtArgs<R> t{};
+        t.m = a;
+        t.rtol = d.rtol;
+        t.atol = d.atol;
+        t.max_steps = p->max_steps;
+        t.hs = (double*)p->hs.p;
+        t.zs = (double*)p->zs.p;
+        t.stats = (long long*)p->sstats.p;
+        t.zsave = d_zsave;
+        t.nz = nz;
+        t.saved = (fh::C2<R>*)d_saved;
+        hipError_t e = fh::launch_tsit5_multi(d.nt, d.modes, t, batch, s);
+        if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode launch: ") + hipGetErrorString(e));
+        return FH_OK;
+    }
     hipError_t e = fh::launch_multi(d.nt, d.modes, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode launch: ") + hipGetErrorString(e));
     return FH_OK;
@@ -537,7 +557,8 @@ int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream
                    int nz = 0, void* d_saved = nullptr)
 {
     const fh_problem_desc& d = p->d;
-    if (d.modes > 1) return launch_forward_multi<R>(p, batch, d_u0, d_uL, s, d_lengths, d_nsteps);
+    if (d.modes > 1)
+        return launch_forward_multi<R>(p, batch, d_u0, d_uL, s, d_lengths, d_nsteps, d_zsave, nz, d_saved);
     if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
     fh::SmArgs<R> a = base_args<R>(p);
     a.u0 = (const fh::C2<R>*)d_u0;
@@ -821,22 +842,21 @@ int fh_forward_saveat(fh_plan p, int32_t batch, const double* u0, double* uL, in
                       double* saved, fh_stats* stats)
 {
     if (!p || !u0 || !uL || !zsave || !saved) return fail(FH_EINVAL, "plan, field or save pointer is NULL");
-    if (!p->adaptive || p->d.modes != 1)
-        return fail(FH_EUNSUPPORTED, "dense output (saveat) needs an adaptive (FH_SCHEME_TSIT5) single-mode plan");
+    if (!p->adaptive) return fail(FH_EUNSUPPORTED, "dense output (saveat) needs an adaptive (FH_SCHEME_TSIT5) plan");
     if (batch < 1 || batch > p->d.max_batch)
         return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
-    const size_t nt = (size_t)p->d.nt;
+    const size_t nt = (size_t)p->d.nt, nm = (size_t)p->d.modes * nt;
     const bool fp64 = p->d.precision == FH_PREC_FP64;
     const size_t esz = fp64 ? 16 : 8;
-    if (nz < 1 || (size_t)nz * nt * esz >= ((size_t)1 << 31))
-        return fail(FH_EINVAL, "nz must lie in [1, 2^31 / (nt * element size))");
+    if (nz < 1 || (size_t)nz * nm * esz >= ((size_t)1 << 31))
+        return fail(FH_EINVAL, "nz must lie in [1, 2^31 / (modes * nt * element size))");
     for (int j = 0; j < nz; ++j) {
         if (!std::isfinite(zsave[j])) return fail(FH_EINVAL, "saveat distances must be finite");
         if (zsave[j] < 0 || zsave[j] > p->d.length_m)
             return fail(FH_EINVAL, "saveat distances must lie in [0, L]");
         if (j > 0 && !(zsave[j] > zsave[j - 1])) return fail(FH_EINVAL, "saveat distances must be sorted and unique");
     }
-    const size_t n = (size_t)batch * nt;
+    const size_t n = (size_t)batch * nm;
     double norm2 = 0.0;
     for (size_t i = 0; i < 2 * n; ++i) {
         if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");

@@ -76,15 +76,13 @@ def propagate(problem: FiberFieldProblem, saveat=None, steps: Optional[int] = No
     Without ``steps`` the integrator is the reference's adaptive Tsit5 with
     the fiber's reltol/abstol (solve_disp_mmf, simulate_disp_mmf.jl:183-207)
     and ``saveat`` samples its continuous extension, as _lab_spectrum samples
-    the ODE solution.  With ``steps`` it is the fixed-step global-frame
-    RK4IP of _scalar_rk4_output (the multimode kernels' integrator), which
-    stores the input and output only."""
+    the ODE solution (any M).  With ``steps`` it is the fixed-step
+    global-frame RK4IP of _scalar_rk4_output, which stores the input and
+    output only."""
     length = float(problem.fiber["L"])
     z_m = _save_positions(saveat, length)
     launch = _forward_input(problem, problem.uω0 if input_spectrum is None else input_spectrum)
     if steps is None:
-        if problem.modes != 1:
-            raise ValueError("multimode propagation runs the fixed-step RK4IP scheme: pass steps")
         plan = plan or Plan.from_problem(problem, precision=precision, scheme="tsit5")
         out, stats, saved = plan.forward_saveat(launch, z_m)
         solver = "Tsit5"

@@ -451,6 +451,23 @@ def test_tsit5_saveat_closed_forms():
         assert _rel(rl.saved[j], np.exp(1j * D * z) * u0) < 1e-13, z
 
 
+def test_tsit5_multimode_copies_reduce_to_scalar():
+    """M = 2 identical copies of the SMF-28 problem with gamma_iiii only: the
+    adaptive multimode solve takes the scalar solve's steps (the RMS error
+    norm over N*M equal entries is the scalar norm) and each mode equals the
+    scalar output.  A norm over N instead of N*M, or a cross-mode term, fails."""
+    sim, fib, u0, (D, g, hR, S, omf) = _smf(nt=256, window=3.0, length=0.1, power=0.1)
+    g2 = np.zeros((2, 2, 2, 2))
+    g2[0, 0, 0, 0] = g2[1, 1, 1, 1] = g[0, 0, 0, 0]
+    multi = oracle.Fiber(np.repeat(D.reshape(-1, 1), 2, axis=1), g2, hR, S, omf)
+    rs = oracle.tsit5_forward(fib, 0.1, u0, rtol=1e-8, atol=1e-6)
+    rm = oracle.tsit5_forward(multi, 0.1, np.repeat(u0, 2, axis=1), rtol=1e-8, atol=1e-6)
+    assert rm.accepted == rs.accepted and rm.rejected == rs.rejected
+    assert np.max(np.abs(rm.hs / rs.hs - 1)) < 1e-12
+    for i in range(2):
+        assert _rel(rm.out[:, i:i + 1], rs.out) < 1e-12
+
+
 def test_tsit5_strict_vs_rk4_crosscheck():
     """test_scalar_crosscheck.jl:58-75: SMF28 0.02 m, 0.02 W, Nt=128, 4 ps;
     128 fixed RK4 steps vs the adaptive solver at (1e-11, 1e-12): < 1e-7,
