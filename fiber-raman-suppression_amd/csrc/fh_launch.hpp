@@ -25,8 +25,9 @@ hipError_t launch_field_metrics(int nt, bool fp64, const void* spec, const void*
 
 template <typename R> struct MtArgs;
 int tsit5_multi_fields(int m);
-hipError_t launch_tsit5_multi(int nt, int m, const MtArgs<double>& a, int batch, hipStream_t s);
-hipError_t launch_tsit5_multi(int nt, int m, const MtArgs<float>& a, int batch, hipStream_t s);
+int tsit5_multi_grad_fields(int m);
+hipError_t launch_tsit5_multi(int nt, int m, bool grad, const MtArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_tsit5_multi(int nt, int m, bool grad, const MtArgs<float>& a, int batch, hipStream_t s);
 
 bool multi_supported(int nt, int m, bool fp64);
 int multi_fields(int m);

@@ -251,10 +251,10 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
 
 // Full right-hand side at zs for the adaptive driver: k_i = f_i(zs, y) of the
 // input fields IN + i into the output fields OUT + i (U, HQ: scratch fields
-// of mm_rhs_head).
-template <typename R, int N, int T, int M>
+// of mm_rhs_head; with TAPE, u and h also go to tape slot `tslot`).
+template <typename R, int N, int T, int M, bool TAPE = false>
 __device__ __forceinline__ void mm_rhs(const MmCtx<R, N, T, M>& c, double zs, int IN, int OUT, int U, int HQ,
-                                       C2<R>* v, void* lds)
+                                       const Buf& tape_u, const Buf& tape_h, uint32_t tslot, C2<R>* v, void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
@@ -264,8 +264,7 @@ __device__ __forceinline__ void mm_rhs(const MmCtx<R, N, T, M>& c, double zs, in
     constexpr int P = TR::P;
     const MmArgs<R>& a = c.a;
     const uint32_t vo = CV::voff();
-    const Buf none(a.u0, 0);
-    mm_rhs_head<R, N, T, M, false>(c, zs, IN, U, HQ, none, none, 0, v, lds);
+    mm_rhs_head<R, N, T, M, TAPE>(c, zs, IN, U, HQ, tape_u, tape_h, tslot, v, lds);
     drain_memory();  // re-reads hq of step 3
     // 4. per mode: eta_i = sum_j h_ij u_j, k_i = i cis(-D_i zs) shock ifft(eta_i)
 #pragma unroll 1

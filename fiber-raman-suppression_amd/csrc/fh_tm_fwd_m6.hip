@@ -1,0 +1,18 @@
+// Adaptive multimode forward kernels for M = 6 (see fh_tsit5_multi_inst.hpp).
+#include "fh_tsit5_multi_inst.hpp"
+
+namespace fh {
+namespace tm {
+
+hipError_t launch_fwd_m6(int nt, const MtArgs<double>& a, int batch, hipStream_t s)
+{
+    return by_size<double, 6, false>(nt, a, batch, s);
+}
+
+hipError_t launch_fwd_m6(int nt, const MtArgs<float>& a, int batch, hipStream_t s)
+{
+    return by_size<float, 6, false>(nt, a, batch, s);
+}
+
+}  // namespace tm
+}  // namespace fh

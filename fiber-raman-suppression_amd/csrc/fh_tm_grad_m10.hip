@@ -1,0 +1,18 @@
+// Adaptive multimode gradient kernels for M = 10 (see fh_tsit5_multi_inst.hpp).
+#include "fh_tsit5_multi_inst.hpp"
+
+namespace fh {
+namespace tm {
+
+hipError_t launch_grad_m10(int nt, const MtArgs<double>& a, int batch, hipStream_t s)
+{
+    return by_size<double, 10, true>(nt, a, batch, s);
+}
+
+hipError_t launch_grad_m10(int nt, const MtArgs<float>& a, int batch, hipStream_t s)
+{
+    return by_size<float, 10, true>(nt, a, batch, s);
+}
+
+}  // namespace tm
+}  // namespace fh

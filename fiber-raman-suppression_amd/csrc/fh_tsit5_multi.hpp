@@ -36,8 +36,28 @@ template <int M> struct MtLayout {
     static constexpr int FIELDS = 10 * M + (M * M + 1) / 2;
 };
 
-template <typename R, int N, int T, int M>
-__global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
+// Gradient slice: the forward phase uses MtLayout; the backward phase keeps
+// y, u, hq below the adjoint fields of MmTraits (lam, lacc, gx, G, gu, bq,
+// which mm_vjp addresses) and the six stage slopes k0..k5 after them.  The
+// stage slopes double as the stage-input cotangents gY_1..gY_5 of the
+// backward sweep; gx holds the stage inputs during the recomputation.
+template <typename R, int N, int T, int M> struct MtgLayout {
+    using TR = MmTraits<R, N, T, M>;
+    static constexpr int Y = 0, U = M, HQ = 2 * M, K0 = TR::FIELDS_GRAD;
+    static constexpr int FIELDS = TR::FIELDS_GRAD + 6 * M;
+    static_assert(HQ + TR::QCMAX <= TR::LAM, "backward scratch must stay below the adjoint fields");
+    static_assert(FIELDS >= MtLayout<M>::FIELDS, "the forward phase shares the slice");
+};
+
+// Adaptive integration from (y group of MtLayout, z = 0) to this solve's L.
+// On return *ys_out is the group holding y(L).  Accepted step sizes and
+// positions are recorded; with CKPT the state at accepted steps 0, every,
+// 2 every, ... is stored to ckpt (M fields per checkpoint); without CKPT the
+// dense-output samples of a.zsave are written.  Returns the workgroup-uniform
+// status: 0 ok, 1 non-finite, 2 step limit.
+template <typename R, int N, int T, int M, bool CKPT>
+__device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const Buf& ckpt, int* ys_out,
+                          long long* nacc_out, long long* nrej_out, long long* nfe_out, C2<R>* v, void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
@@ -46,23 +66,16 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
     using CV = Cyc<V, T>;
     using CD = Cyc<double, T>;
     constexpr int P = TR::P;
-    __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
     const size_t sol = blockIdx.x;
     const MmArgs<R>& ma = a.m;
-    V* slice = ma.work + sol * (size_t)LY::FIELDS * N;
-    const Buf sl(slice, (uint32_t)((size_t)LY::FIELDS * N * sizeof(V)));
-    const Buf disp(ma.disp, M * N * 8), hn(ma.hn, N * sizeof(V)), shn(ma.shn, N * sizeof(R));
-    const Buf tw(ma.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
-    const Buf in(ma.u0 + sol * M * N, M * N * sizeof(V));
-    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, ma.q, (ma.q + 1) / 2};
     const uint32_t vo = CV::voff();
     const uint32_t FB = (uint32_t)(N * sizeof(V));
     const double L = ma.lengths ? ma.lengths[sol] : ma.length;
     const double rtol = a.rtol, atol = a.atol;
     const double nm = (double)N * M;
+    const Buf none(ma.u0, 0);
     int sy = LY::Y, syn = LY::YN, sk0 = LY::K0, sk6 = LY::K0 + 6 * M;
     auto kslot = [&](int j) { return j == 0 ? sk0 : (j == 6 ? sk6 : LY::K0 + j * M); };
-    V v[P];
     long long nfe = 0, nacc = 0, nrej = 0;
 
     // dense output: lab-frame sample of every mode at zj (see ts_save)
@@ -94,21 +107,14 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
                     val = V{u.x + hr * acc.x, u.y + hr * acc.y};
                 }
                 R cs, sn;
-                cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zj, cs, sn);
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zj, cs, sn);
                 dst.st(V{val.x * cs - val.y * sn, val.x * sn + val.y * cs}, vo, CV::soff(k));
             }
         }
     };
 
-#pragma unroll 1
-    for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(sy + m);
-#pragma unroll
-        for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
-    }
-
     // k0 = f(0, y0) and the starting-step norms (Hairer-Norsett-Wanner II.4)
-    mm_rhs<R, N, T, M>(c, 0.0, sy, sk0, LY::U, LY::HQ, v, lds);
+    mm_rhs<R, N, T, M>(c, 0.0, sy, sk0, LY::U, LY::HQ, none, none, 0, v, lds);
     nfe++;
     drain_memory();
     double s0 = 0, s1 = 0;
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
             yn.st(V{u.x + (R)dt0 * f0.x, u.y + (R)dt0 * f0.y}, vo, CV::soff(k));
         }
     }
-    mm_rhs<R, N, T, M>(c, dt0, syn, kslot(1), LY::U, LY::HQ, v, lds);
+    mm_rhs<R, N, T, M>(c, dt0, syn, kslot(1), LY::U, LY::HQ, none, none, 0, v, lds);
     nfe++;
     drain_memory();
     double s2 = 0;
@@ -158,16 +164,29 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
     double h = fmin(fmin(100 * dt0, dt1), L);
 
     double z = 0.0, qold = TB::qoldinit;
-    int status = 0, js = 0;
-    while (js < a.nz && a.zsave[js] == 0.0) {
-        save(js, 0.0, 0.0, 0.0, 0.0);
-        ++js;
+    int status = 0, js = 0, last_ck = -1;
+    if constexpr (!CKPT) {
+        while (js < a.nz && a.zsave[js] == 0.0) {
+            save(js, 0.0, 0.0, 0.0, 0.0);
+            ++js;
+        }
     }
     bool done = false;
     while (!done) {
         if (nacc >= a.max_steps || nrej >= a.max_steps) {  // bounded for every wave
             status = 2;
             break;
+        }
+        if (CKPT && nacc % ma.every == 0 && last_ck != (int)nacc) {
+            drain_memory();
+#pragma unroll 1
+            for (int m = 0; m < M; ++m) {
+                const Fld y = c.fld(sy + m);
+                const Fld ck{ckpt, ((uint32_t)(nacc / ma.every) * (uint32_t)M + (uint32_t)m) * FB};
+#pragma unroll
+                for (int k = 0; k < P; ++k) ck.st(y.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+            }
+            last_ck = (int)nacc;
         }
         bool last = false;
         if (z + h >= L) {
@@ -196,7 +215,7 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
                     yn.st(V{u.x + hr * s.x, u.y + hr * s.y}, vo, CV::soff(k));
                 }
             }
-            mm_rhs<R, N, T, M>(c, z + TB::c[i] * h, syn, kslot(i), LY::U, LY::HQ, v, lds);
+            mm_rhs<R, N, T, M>(c, z + TB::c[i] * h, syn, kslot(i), LY::U, LY::HQ, none, none, 0, v, lds);
         }
         nfe += 6;
         drain_memory();
@@ -230,10 +249,12 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
                 a.hs[sol * a.max_steps + nacc] = h;
                 a.zs[sol * a.max_steps + nacc] = z;
             }
-            const double zend = last ? L : z + h;
-            while (js < a.nz && a.zsave[js] <= zend) {
-                save(js, a.zsave[js], z, h, zend);
-                ++js;
+            if constexpr (!CKPT) {
+                const double zend = last ? L : z + h;
+                while (js < a.nz && a.zsave[js] <= zend) {
+                    save(js, a.zsave[js], z, h, zend);
+                    ++js;
+                }
             }
             nacc++;
             z += h;
@@ -250,6 +271,45 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
             h = h / fmin(1.0 / TB::qmin, q11 / TB::safety);
         }
     }
+    *ys_out = sy;
+    *nacc_out = nacc;
+    *nrej_out = nrej;
+    *nfe_out = nfe;
+    return status;
+}
+
+template <typename R, int N, int T, int M>
+__global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using LY = MtLayout<M>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = TR::P;
+    __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
+    const size_t sol = blockIdx.x;
+    const MmArgs<R>& ma = a.m;
+    V* slice = ma.work + sol * (size_t)LY::FIELDS * N;
+    const Buf sl(slice, (uint32_t)((size_t)LY::FIELDS * N * sizeof(V)));
+    const Buf disp(ma.disp, M * N * 8), hn(ma.hn, N * sizeof(V)), shn(ma.shn, N * sizeof(R));
+    const Buf tw(ma.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf in(ma.u0 + sol * M * N, M * N * sizeof(V));
+    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, ma.q, (ma.q + 1) / 2};
+    const uint32_t vo = CV::voff();
+    const uint32_t FB = (uint32_t)(N * sizeof(V));
+    const double L = ma.lengths ? ma.lengths[sol] : ma.length;
+    V v[P];
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld y = c.fld(LY::Y + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
+    }
+    int ys;
+    long long nacc, nrej, nfe;
+    const Buf none(ma.u0, 0);
+    const int status = mt_forward<R, N, T, M, false>(a, c, none, &ys, &nacc, &nrej, &nfe, v, lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
@@ -260,7 +320,7 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
     const Buf out(ma.out + sol * M * N, M * N * sizeof(V));
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(sy + m);
+        const Fld y = c.fld(ys + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             R cs, sn;
@@ -272,6 +332,258 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
         }
     }
     if (bad) atomicOr(ma.status + sol, bad);
+}
+
+// Gradient of the terminal objective through the discrete adjoint of the
+// accepted-step map (the multimode counterpart of k_grad_tsit5): forward with
+// checkpoints, output and seed (band objective summed over modes, mmf_cost_sum,
+// or the caller's seed), then per segment a recomputation with the six stage
+// tapes (u and h pairs) and the backward sweep
+//   gk_i = h b_i lam' + h sum_{m > i} a_mi gY_m,   gY_i = J_i^T gk_i,
+//   lam  = lam' + sum_i gY_i,
+// with J_i^T the multimode pullback mm_vjp; finally lam0 and the shared-phase
+// gradient sum_m 2 Re(conj(lam0_m) i u0_m) in FP64.
+template <typename R, int N, int T, int M>
+__global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using LF = MtLayout<M>;
+    using LB = MtgLayout<R, N, T, M>;
+    using TB = Tsit5Tab;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = TR::P;
+    __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
+    const size_t sol = blockIdx.x;
+    const MmArgs<R>& ma = a.m;
+    const uint32_t FB = (uint32_t)(N * sizeof(V));
+    V* slice = ma.work + sol * (size_t)LB::FIELDS * N;
+    const Buf sl(slice, (uint32_t)((size_t)LB::FIELDS * N * sizeof(V)));
+    const Buf disp(ma.disp, M * N * 8), hn(ma.hn, N * sizeof(V)), shn(ma.shn, N * sizeof(R));
+    const Buf tw(ma.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf in(ma.u0 + sol * M * N, M * N * sizeof(V));
+    const int Q = ma.q, QC = (ma.q + 1) / 2;
+    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, Q, QC};
+    const Buf ckpt(ma.ckpt + sol * (size_t)ma.nseg * M * N, (uint32_t)((size_t)ma.nseg * M * N * sizeof(V)));
+    const Buf tape_u(ma.tape_u + sol * (size_t)6 * ma.every * M * N,
+                     (uint32_t)((size_t)6 * ma.every * M * N * sizeof(V)));
+    const Buf tape_h(ma.tape_h + sol * (size_t)6 * ma.every * QC * N,
+                     (uint32_t)((size_t)6 * ma.every * QC * N * sizeof(V)));
+    const uint32_t vo = CV::voff();
+    const double L = ma.lengths ? ma.lengths[sol] : ma.length;
+    V v[P];
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld y = c.fld(LF::Y + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
+    }
+    int ys;
+    long long nacc, nrej, nfe;
+    int bad = mt_forward<R, N, T, M, true>(a, c, ckpt, &ys, &nacc, &nrej, &nfe, v, lds);
+    if (threadIdx.x == 0) {
+        a.stats[sol * 3 + 0] = nacc;
+        a.stats[sol * 3 + 1] = nrej;
+        a.stats[sol * 3 + 2] = nfe;
+    }
+    if (bad) {
+        if (threadIdx.x == 0) atomicOr(ma.status + sol, bad);
+        return;  // workgroup-uniform
+    }
+    drain_memory();
+    // output and terminal seed (band energy over all modes)
+    const Buf out(ma.out + sol * M * N, M * N * sizeof(V));
+    double etot = 0.0, eband = 0.0;
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld y = c.fld(ys + m), lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R cs, sn;
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), L, cs, sn);
+            const V x = y.template ld<V>(vo, CV::soff(k));
+            const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+            bad |= !(isfinite(o.x) && isfinite(o.y));
+            out.st(o, vo, (uint32_t)m * FB + CV::soff(k));
+            lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
+            const double e = (double)o.x * o.x + (double)o.y * o.y;
+            etot += e;
+            if (!ma.seed && ma.mask[threadIdx.x + T * k]) eband += e;
+        }
+    }
+    double J = 0.0, scale = 0.0;
+    if (!ma.seed) {
+        etot = mm_block_sum<T>(etot, lds);
+        eband = mm_block_sum<T>(eband, lds);
+        J = eband / etot;
+        scale = 1.0 / etot;
+        if (ma.log_cost) scale *= J <= 1e-15 ? 0.0 : 10.0 / (2.302585092994045684 * J);
+        if (threadIdx.x == 0) ma.cost[sol] = ma.log_cost ? 10.0 * log10(fmax(J, 1e-15)) : J;
+    }
+    drain_memory();
+    // lam = cis(-D L) seed
+    const Buf sd(ma.seed ? ma.seed + sol * M * N : ma.u0, ma.seed ? (uint32_t)(M * N * sizeof(V)) : 0u);
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            V s;
+            if (ma.seed) {
+                s = sd.ld<V>(vo, (uint32_t)m * FB + CV::soff(k));
+            } else {
+                const V o = lam.template ld<V>(vo, CV::soff(k));
+                const R f = (R)(((ma.mask[threadIdx.x + T * k] ? 1.0 : 0.0) - J) * scale);
+                s = V{o.x * f, o.y * f};
+            }
+            R cs, sn;
+            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), L, cs, sn);
+            lam.st(V{cs * s.x + sn * s.y, cs * s.y - sn * s.x}, vo, CV::soff(k));
+        }
+    }
+    const int nseg = (int)((nacc + ma.every - 1) / ma.every);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * ma.every;
+        const int s1 = (long long)s0 + ma.every < nacc ? s0 + ma.every : (int)nacc;
+        drain_memory();
+#pragma unroll 1
+        for (int m = 0; m < M; ++m) {
+            const Fld ck{ckpt, ((uint32_t)sg * (uint32_t)M + (uint32_t)m) * FB};
+            const Fld y = c.fld(LB::Y + m);
+#pragma unroll
+            for (int k = 0; k < P; ++k) y.st(ck.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+        }
+        // recompute the segment with stage tapes; stage inputs in gx
+        for (int s = s0; s < s1; ++s) {
+            const double h = a.hs[sol * a.max_steps + s], z = a.zs[sol * a.max_steps + s];
+            const R hr = (R)h;
+#pragma unroll 1
+            for (int i = 0; i < 6; ++i) {
+                if (i > 0) {
+                    drain_memory();
+#pragma unroll 1
+                    for (int m = 0; m < M; ++m) {
+                        const Fld y = c.fld(LB::Y + m), st = c.fld(TR::GX + m);
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            V acc = V{(R)0, (R)0};
+#pragma unroll
+                            for (int j = 0; j < 5; ++j) {
+                                if (j < i) {
+                                    const R aij = (R)TB::a[i][j];
+                                    const V kj = c.fld(LB::K0 + j * M + m).template ld<V>(vo, CV::soff(k));
+                                    acc = V{acc.x + aij * kj.x, acc.y + aij * kj.y};
+                                }
+                            }
+                            const V u = y.template ld<V>(vo, CV::soff(k));
+                            st.st(V{u.x + hr * acc.x, u.y + hr * acc.y}, vo, CV::soff(k));
+                        }
+                    }
+                }
+                mm_rhs<R, N, T, M, true>(c, z + TB::c[i] * h, i == 0 ? LB::Y : TR::GX, LB::K0 + i * M, LB::U,
+                                         LB::HQ, tape_u, tape_h, (uint32_t)(6 * (s - s0) + i), v, lds);
+            }
+            if (s + 1 < s1) {
+                drain_memory();
+#pragma unroll 1
+                for (int m = 0; m < M; ++m) {
+                    const Fld y = c.fld(LB::Y + m);
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        V acc = V{(R)0, (R)0};
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) {
+                            const V kj = c.fld(LB::K0 + j * M + m).template ld<V>(vo, CV::soff(k));
+                            acc = V{acc.x + (R)TB::a[6][j] * kj.x, acc.y + (R)TB::a[6][j] * kj.y};
+                        }
+                        const V u = y.template ld<V>(vo, CV::soff(k));
+                        y.st(V{u.x + hr * acc.x, u.y + hr * acc.y}, vo, CV::soff(k));
+                    }
+                }
+            }
+        }
+        // backward over the segment: gY_i kept in the k slots (i >= 1)
+        for (int s = s1 - 1; s >= s0; --s) {
+            const double h = a.hs[sol * a.max_steps + s], z = a.zs[sol * a.max_steps + s];
+            const R hr = (R)h;
+            drain_memory();
+#pragma unroll 1
+            for (int m = 0; m < M; ++m) {
+                const Fld lam = c.fld(TR::LAM + m), lacc = c.fld(TR::LACC + m);
+#pragma unroll
+                for (int k = 0; k < P; ++k) lacc.st(lam.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+            }
+#pragma unroll 1
+            for (int i = 5; i >= 0; --i) {
+                // gx = h sum_{m > i} a_mi gY_m (the h b_i lam' part enters through cl)
+                if (i < 5) {
+                    drain_memory();
+#pragma unroll 1
+                    for (int m = 0; m < M; ++m) {
+                        const Fld gx = c.fld(TR::GX + m);
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            V g = V{(R)0, (R)0};
+#pragma unroll
+                            for (int r = 1; r < 6; ++r) {
+                                if (r > i) {
+                                    const R ari = hr * (R)TB::a[r][i];
+                                    const V gy = c.fld(LB::K0 + r * M + m).template ld<V>(vo, CV::soff(k));
+                                    g = V{g.x + ari * gy.x, g.y + ari * gy.y};
+                                }
+                            }
+                            gx.st(g, vo, CV::soff(k));
+                        }
+                    }
+                }
+                mm_vjp<R, N, T, M>(c, z + TB::c[i] * h, hr * (R)TB::a[6][i], i < 5 ? (R)1 : (R)0, tape_u, tape_h,
+                                   (uint32_t)(6 * (s - s0) + i), v, lds);
+                drain_memory();
+#pragma unroll 1
+                for (int m = 0; m < M; ++m) {
+                    const Fld gx = c.fld(TR::GX + m), lacc = c.fld(TR::LACC + m);
+                    const Fld gi = c.fld(LB::K0 + i * M + m);
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const V g = gx.template ld<V>(vo, CV::soff(k));
+                        if (i >= 1) gi.st(g, vo, CV::soff(k));
+                        const V acc = lacc.template ld<V>(vo, CV::soff(k));
+                        lacc.st(V{acc.x + g.x, acc.y + g.y}, vo, CV::soff(k));
+                    }
+                }
+            }
+            drain_memory();
+#pragma unroll 1
+            for (int m = 0; m < M; ++m) {
+                const Fld lam = c.fld(TR::LAM + m), lacc = c.fld(TR::LACC + m);
+#pragma unroll
+                for (int k = 0; k < P; ++k) lam.st(lacc.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+            }
+        }
+    }
+    drain_memory();
+    // lam0 and the shared-phase gradient in FP64
+    const Buf l0(ma.lam0 + sol * M * N, M * N * sizeof(V));
+    const Buf gb(ma.grad + sol * N, N * sizeof(double));
+    double g[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) g[k] = 0.0;
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+        const Fld lam = c.fld(TR::LAM + m);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const V l = lam.template ld<V>(vo, CV::soff(k));
+            const V u = in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k));
+            l0.st(l, vo, (uint32_t)m * FB + CV::soff(k));
+            g[k] += 2.0 * ((double)l.y * (double)u.x - (double)l.x * (double)u.y);
+            bad |= !(isfinite(l.x) && isfinite(l.y));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) gb.st(g[k], CD::voff(), CD::soff(k));
+    if (bad) atomicOr(ma.status + sol, 1);
 }
 
 }  // namespace fh

@@ -419,7 +419,8 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
     int rc;
     if (M > 1) {  // the gradient kernel's slice holds the adjoint fields too
         total_release(p, p->work);
-        if ((rc = dev_alloc(p, p->work, V * (size_t)fh::multi_grad_fields((int)M) * nt * mb))) return rc;
+        const int f = p->adaptive ? fh::tsit5_multi_grad_fields((int)M) : fh::multi_grad_fields((int)M);
+        if ((rc = dev_alloc(p, p->work, V * (size_t)f * nt * mb))) return rc;
     }
     if ((rc = dev_alloc(p, p->ckpt, V * nt * M * (size_t)p->nseg * mb))) return rc;
     const size_t stages = p->adaptive ? 6 : 4;
@@ -444,8 +445,8 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
     if (p->d.modes > 1) {
         if (!p->gamma_full_sym)
             return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
-        if (!d_lengths != !d_nsteps) return fail(FH_EINVAL, "multimode per-solve lengths need per-solve step counts");
-        if (p->adaptive) return fail(FH_EUNSUPPORTED, "multimode gradients use the fixed-step scheme");
+        if (!p->adaptive && !d_lengths != !d_nsteps)
+            return fail(FH_EINVAL, "multimode per-solve lengths need per-solve step counts");
     }
     int rc = ensure_adjoint_buffers<R>(p);
     if (rc) return rc;
@@ -481,7 +482,23 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
         m.lengths = d_lengths;
         m.nsteps = d_nsteps;  // each at most the plan's steps (checkpoint capacity)
         FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
-        hipError_t e = fh::launch_multi_grad(d.nt, d.modes, m, batch, s);
+        hipError_t e;
+        if (p->adaptive) {
+            m.h = 0.0;
+            m.steps = 0;
+            m.nsteps = nullptr;
+            fh::MtArgs<R> t{};
+            t.m = m;
+            t.rtol = d.rtol;
+            t.atol = d.atol;
+            t.max_steps = p->max_steps;
+            t.hs = (double*)p->hs.p;
+            t.zs = (double*)p->zs.p;
+            t.stats = (long long*)p->sstats.p;
+            e = fh::launch_tsit5_multi(d.nt, d.modes, true, t, batch, s);
+        } else {
+            e = fh::launch_multi_grad(d.nt, d.modes, m, batch, s);
+        }
         if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode gradient launch: ") + hipGetErrorString(e));
         return FH_OK;
     }
@@ -542,7 +559,7 @@ int launch_forward_multi(fh_plan p, int batch, const void* d_u0, void* d_uL, hip
         t.zsave = d_zsave;
         t.nz = nz;
         t.saved = (fh::C2<R>*)d_saved;
-        hipError_t e = fh::launch_tsit5_multi(d.nt, d.modes, t, batch, s);
+        hipError_t e = fh::launch_tsit5_multi(d.nt, d.modes, false, t, batch, s);
         if (e != hipSuccess) return fail(FH_EDEVICE, std::string("multimode launch: ") + hipGetErrorString(e));
         return FH_OK;
     }

@@ -5,7 +5,9 @@ right-hand side (the adaptive solver and the multimode RHS are each pinned in
 tests/test_oracle_pins.py; the copies-of-a-scalar reduction below pins the
 N*M error norm).  FP64: same accepted and rejected step counts as the
 oracle, per-mode output energy within 1e-10, saved spectra within 1e-10.
-FP32-bulk: per-mode energy within 1e-5 of FP64.
+FP32-bulk: per-mode energy within 1e-5 of FP64.  The adaptive multimode
+gradient: lam0, cost and gradient within 1e-10 of the oracle's discrete
+adjoint of the same accepted steps (FP64); FP32 cost within 1e-5.
 """
 from __future__ import annotations
 
@@ -114,5 +116,65 @@ def test_propagate_multimode_default_is_adaptive(gpu_context):
     ev = fiberhip.metrics(res)
     assert ev["max_photon_number_drift"] < 1e-6
     assert fiberhip.verify(res)["pass_"]
-    with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
-        fiberhip.Plan.from_problem(gp, scheme="tsit5", band_mask=np.ones(1024, bool)).gradient(gp.uω0)
+
+
+def _small_grin(modes, nt, length=0.01, energy=2e-9, seed=1):
+    from fiberhip import grin
+
+    window = max(3.0, 1.2 * nt / (2 * 193.41449))
+    return grin.grin_problem(modes, nt=nt, time_window_ps=window, length_m=length, energy_j=energy, nx=96,
+                             window_um=60.0, seed=seed)
+
+
+@pytest.mark.parametrize("modes,nt", [(3, 512), (6, 256)])
+def test_fp64_adaptive_gradient_matches_oracle_adjoint_of_same_steps(gpu_context, modes, nt):
+    """The adaptive multimode gradient is the discrete adjoint of the GPU's own
+    accepted-step map: lam0, cost and the shared-phase gradient within 1e-10
+    of the oracle's adjoint of that step sequence (orc_tsit5_adjoint, pinned
+    against central differences for the scalar case and by the multimode
+    RHS pullback pins)."""
+    import fiberhip
+    import oracle
+    from oracle import np_ref as R
+
+    p = _small_grin(modes, nt)
+    mask = fiberhip.frequency_band_mask(p, (-16.0, -4.0))
+    L = p.fiber["L"]
+    print(f"case: adaptive multimode gradient fp64 M={modes} nt={nt}", flush=True)
+    plan = fiberhip.Plan.from_problem(p, scheme="tsit5", rtol=1e-8, atol=1e-6, band_mask=mask, checkpoint_every=7)
+    res = plan.gradient(p.uω0)
+    hs = plan.step_sizes(1)[0]
+    assert hs.size > 7
+    fib = oracle_fiber(p)
+    uL = oracle.tsit5_replay(fib, L, hs, p.uω0)
+    J, seed = R.band_cost(uL, mask)
+    lam0 = oracle.tsit5_adjoint(fib, L, hs, p.uω0, seed, every=50)
+    grad = np.sum(2 * np.real(np.conj(lam0) * 1j * p.uω0), axis=1)
+    print("steps", hs.size, "lam0 rel", rel(res["lam0"], lam0), "cost rel", abs(res["cost"] / J - 1), flush=True)
+    assert rel(res["uL"], uL) <= 1e-10
+    assert abs(res["cost"] - J) <= 1e-11 * J
+    assert rel(res["lam0"], lam0) <= 1e-10
+    assert rel(res["grad"], grad) <= 1e-10
+    # the forward solve takes the same steps
+    out, st = fiberhip.Plan.from_problem(p, scheme="tsit5", rtol=1e-8, atol=1e-6).forward(p.uω0)
+    assert st["accepted"] == hs.size and rel(out, res["uL"]) <= 1e-13
+
+
+def test_adaptive_gradient_fp32_and_identical_members(gpu_context):
+    import fiberhip
+
+    p = _small_grin(4, 1024, length=0.02, energy=3e-9)
+    mask = fiberhip.frequency_band_mask(p, (-16.0, -4.0))
+    res = {}
+    for precision in ("fp64", "fp32"):
+        print(f"case: adaptive multimode gradient identical members {precision} M=4", flush=True)
+        plan = fiberhip.Plan.from_problem(p, scheme="tsit5", rtol=1e-7, atol=1e-6, band_mask=mask,
+                                          precision=precision, max_batch=4)
+        r = plan.gradient(np.stack([p.uω0] * 4))
+        for key in ("lam0", "grad", "uL", "cost"):
+            assert all(np.array_equal(r[key][b], r[key][0]) for b in range(4)), (precision, key)
+        res[precision] = r
+    cost_err = abs(res["fp32"]["cost"][0] / res["fp64"]["cost"][0] - 1)
+    grad_err = rel(res["fp32"]["grad"][0], res["fp64"]["grad"][0])
+    print("fp32 vs fp64 cost rel", cost_err, "grad normwise", grad_err, flush=True)
+    assert cost_err <= 1e-5 and grad_err <= 1e-2
