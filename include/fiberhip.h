@@ -64,8 +64,10 @@ enum { FH_PREC_FP64 = 0, FH_PREC_FP32_BULK = 1 };
  * RK4 of _scalar_rk4_output (scalar_crosscheck.jl:53-60), z_i = i*h, with
  * `steps` steps.  FH_SCHEME_TSIT5 is the adaptive Tsit5 (Tsitouras 5(4)) of
  * solve_disp_mmf (simulate_disp_mmf.jl:183-207) with relative/absolute
- * tolerances rtol/atol, an FP64 RMS error norm and PI step control; every
- * solve of a batch runs its own step sequence (single mode). */
+ * tolerances rtol/atol, an FP64 RMS error norm over all nt*modes entries and
+ * PI step control; every solve of a batch runs its own step sequence.  Both
+ * schemes cover the single-mode grids and the multimode grids and mode
+ * counts of the library (FH_EUNSUPPORTED otherwise). */
 enum { FH_SCHEME_RK4IP_FIXED = 0, FH_SCHEME_TSIT5 = 1 };
 
 typedef struct fh_context_s *fh_context;
@@ -130,7 +132,7 @@ int fh_forward(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_sta
  * step boundary return the stored state; positions inside a step use the
  * solver's 4th-order continuous extension.  saved: batch*nz*modes*nt
  * complex128, ordered [batch][z][mode][t].  Adaptive (FH_SCHEME_TSIT5)
- * single-mode plans only (FH_EUNSUPPORTED otherwise). */
+ * plans only, any supported mode count (FH_EUNSUPPORTED otherwise). */
 int fh_forward_saveat(fh_plan plan, int32_t batch, const double *u0, double *uL, int32_t nz, const double *zsave,
                       double *saved, fh_stats *stats);
 
@@ -140,7 +142,9 @@ int fh_forward_saveat(fh_plan plan, int32_t batch, const double *u0, double *uL,
 int fh_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *stream);
 
 /* Gradient of a terminal objective through the discrete adjoint of the
- * fixed-step RK4IP map (the role of solve_adjoint_disp_mmf plus the chain
+ * plan's integrator -- the fixed-step RK4IP map, or the adaptive solve's
+ * accepted-step map with its step sizes held fixed -- (the role of
+ * solve_adjoint_disp_mmf plus the chain
  * rule of spectral_shaper_model, sensitivity_disp_mmf.jl:309-332,
  * physics_models.jl:1042-1046).  For each of `batch` launches u0 (host,
  * complex128, the already-shaped spectrum alpha*A*cis(phi)*uw0):
