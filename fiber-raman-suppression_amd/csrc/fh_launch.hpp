@@ -14,6 +14,15 @@ template <typename R, int N> struct SingleShape {
                                             : (N >= 32768 ? 1024 : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4)));
 };
 
+// Multimode kernels: FP32 at 4096 samples takes 8 samples per thread (512
+// threads, twice the single-mode workgroup).  A multimode solve keeps its
+// per-sample work in global scratch and needs more waves per CU to hide that
+// latency; 512 threads still leave 256 VGPRs per wave for the M = 10
+// contraction.  Other sizes and FP64 use the single-mode shape.
+template <typename R, int N> struct MultiShape {
+    static constexpr int T = (sizeof(R) == 4 && N == 4096) ? 512 : SingleShape<R, N>::T;
+};
+
 template <typename R> struct TsArgs;
 bool tsit5_supported(int nt, int m, bool fp64);
 int tsit5_fields(int nt, bool fp64);

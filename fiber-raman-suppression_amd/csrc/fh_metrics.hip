@@ -112,10 +112,16 @@ __global__ __launch_bounds__(T) void k_field_metrics(MetricArgs<R> a)
     }
 }
 
+// The transform must match the plan's twiddle table: single-mode plans use
+// the single-mode workgroup shape, multimode plans the multimode one.
 template <typename R, int N> hipError_t metrics_go(const MetricArgs<R>& a, int count, hipStream_t s)
 {
-    constexpr int T = SingleShape<R, N>::T;
-    hipLaunchKernelGGL((k_field_metrics<R, N, T>), dim3(count), dim3(T), 0, s, a);
+    constexpr int TS = SingleShape<R, N>::T, TM = MultiShape<R, N>::T;
+    if (a.modes == 1 || TS == TM) {
+        hipLaunchKernelGGL((k_field_metrics<R, N, TS>), dim3(count), dim3(TS), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_field_metrics<R, N, TM>), dim3(count), dim3(TM), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -13,7 +13,7 @@ template <typename R, int N, int M> hipError_t go(bool grad, const fh::MmArgs<R>
     if constexpr (sizeof(R) == 8 && M == 2) {
         return hipErrorInvalidValue;  // not offered (multi_supported)
     } else {
-        constexpr int T = fh::SingleShape<R, N>::T;
+        constexpr int T = fh::MultiShape<R, N>::T;
         if (grad)
             hipLaunchKernelGGL((fh::k_grad_multi<R, N, T, M>), dim3(batch), dim3(T), 0, s, a);
         else

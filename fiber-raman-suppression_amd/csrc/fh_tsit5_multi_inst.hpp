@@ -15,7 +15,7 @@ template <typename R, int N, int M, bool GRAD> hipError_t go(const MtArgs<R>& a,
     if constexpr (sizeof(R) == 8 && M == 2) {
         return hipErrorInvalidValue;  // not offered (multi_supported)
     } else {
-        constexpr int T = SingleShape<R, N>::T;
+        constexpr int T = MultiShape<R, N>::T;
         if constexpr (GRAD)
             hipLaunchKernelGGL((k_grad_tsit5_multi<R, N, T, M>), dim3(batch), dim3(T), 0, s, a);
         else

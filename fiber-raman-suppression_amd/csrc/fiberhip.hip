@@ -45,7 +45,7 @@ struct DevBuf {
 struct fh_plan_s {
     fh_context ctx;
     fh_problem_desc d;
-    int threads;  // workgroup size of the single-mode kernel
+    int threads;  // workgroup size of the plan's kernels (single-mode or multimode shape)
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
     int every = 50, nseg = 0;
@@ -166,6 +166,14 @@ template <typename R, int N> struct OpTw {
     }
     static int unsupported() { return 0; }
 };
+template <typename R, int N> struct OpTwMulti {
+    static int run(std::vector<std::complex<long double>>& tw)
+    {
+        fill_twiddles<N, fh::MultiShape<R, N>::T>(tw);
+        return fh::MultiShape<R, N>::T;
+    }
+    static int unsupported() { return 0; }
+};
 template <typename R, int N> struct OpGrad {
     static hipError_t run(const fh::SmArgs<R>& a, int batch, hipStream_t s) { return GradKernel<R, N>::launch(a, batch, s); }
     static hipError_t unsupported() { return hipErrorInvalidValue; }
@@ -262,7 +270,7 @@ template <typename R> int upload_tables(fh_plan p)
     const fh_problem_desc& d = p->d;
     const int nt = d.nt;
     std::vector<std::complex<long double>> tw;
-    const int T = with_size<R, OpTw>(nt, tw);
+    const int T = d.modes == 1 ? with_size<R, OpTw>(nt, tw) : with_size<R, OpTwMulti>(nt, tw);
     if (T == 0) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(nt));
     p->threads = T;
     std::vector<R> twr(2 * tw.size()), hn(2 * (size_t)nt), shn(nt);
