@@ -33,6 +33,8 @@
 // in the cyclic layout through one fixed descriptor, so outside the
 // transforms every thread only touches its own samples.
 #pragma once
+#include <type_traits>
+
 #include "fh_device.hpp"
 
 namespace fh {
@@ -80,6 +82,15 @@ template <typename R, int N, int T, int M> struct MmTraits {
     static constexpr int LAM = FIELDS, LACC = LAM + M, GX = LACC + M, GG = GX + M, GU = GG + M, BQ = GU + M;
     static constexpr int FIELDS_GRAD = BQ + QCMAX;
 };
+
+// Output-pair index q of (i, j), in the order the plan builds qidx: the
+// pairs i <= j in row order when gamma is symmetric in (i, j) (h_ij = h_ji),
+// otherwise i M + j.  With i, j unrolled constants it indexes register
+// arrays at compile time.
+__host__ __device__ constexpr int pair_index(int M, int i, int j, bool sym)
+{
+    return sym ? (i <= j ? i * M - i * (i - 1) / 2 + (j - i) : j * M - j * (j - 1) / 2 + (i - j)) : i * M + j;
+}
 
 // Per-workgroup views shared by the stage and pullback routines.
 template <typename R, int N, int T, int M> struct MmCtx {
@@ -184,6 +195,48 @@ __device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double z
     }
 }
 
+// eta_i = sum_j h_ij u_j for every mode, per sample: u and the h pairs are
+// loaded once per sample (M + QC loads instead of 2 M^2) and eta_i overwrites
+// u_i in place (u is not read again by the stage; the tape holds its copy).
+template <typename R, int N, int T, int M, bool SYM>
+__device__ __forceinline__ void mm_eta_body(const MmCtx<R, N, T, M>& c, int U, int HQ)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    constexpr int P = TR::P;
+    constexpr int QS = SYM ? (TR::NP + 1) / 2 : TR::QCMAX;
+    const uint32_t vo = CV::voff();
+#pragma unroll 1
+    for (int k = 0; k < P; ++k) {
+        V u[M], hp[QS];
+#pragma unroll
+        for (int m = 0; m < M; ++m) u[m] = c.fld(U + m).template ld<V>(vo, CV::soff(k));
+#pragma unroll
+        for (int cc = 0; cc < QS; ++cc) hp[cc] = c.fld(HQ + cc).template ld<V>(vo, CV::soff(k));
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            V acc = V{(R)0, (R)0};
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const int q = pair_index(M, i, j, SYM);
+                const R hij = (q & 1) ? hp[q >> 1].y : hp[q >> 1].x;
+                acc = V{acc.x + hij * u[j].x, acc.y + hij * u[j].y};
+            }
+            c.fld(U + i).st(acc, vo, CV::soff(k));
+        }
+    }
+}
+
+template <typename R, int N, int T, int M>
+__device__ __forceinline__ void mm_eta_inplace(const MmCtx<R, N, T, M>& c, int U, int HQ)
+{
+    if (c.Q != M * M)
+        mm_eta_body<R, N, T, M, true>(c, U, HQ);
+    else
+        mm_eta_body<R, N, T, M, false>(c, U, HQ);
+}
+
 // One RK4 stage s of the step from state st at position zs (the right-hand
 // side fused with the RK4 bookkeeping of y/st/ac).  Tape as in mm_rhs_head.
 template <typename R, int N, int T, int M, bool TAPE>
@@ -201,22 +254,14 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
     const R h2 = (R)(h / 2), h1 = (R)h, h6 = (R)(h / 6);
     mm_rhs_head<R, N, T, M, TAPE>(c, zs, 0, 3 * M, 4 * M, tape_u, tape_h, tslot, v, lds);
     drain_memory();  // re-reads hq of step 3
-    // 4. per mode: eta_i = sum_j h_ij u_j; k_i; RK4 bookkeeping
+    // 4. eta_i = sum_j h_ij u_j (into the u fields); per mode: k_i; RK4 bookkeeping
+    mm_eta_inplace<R, N, T, M>(c, 3 * M, 4 * M);
+    drain_memory();  // re-reads eta
 #pragma unroll 1
     for (int i = 0; i < M; ++i) {
+        const Fld eta = c.fld(3 * M + i);
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            V acc = V{(R)0, (R)0};
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                const int q = a.qidx[i * M + j];
-                const V hp = c.fld(4 * M + (q >> 1)).template ld<V>(vo, CV::soff(k));
-                const R hij = (q & 1) ? hp.y : hp.x;
-                const V uj = c.fld(3 * M + j).template ld<V>(vo, CV::soff(k));
-                acc = V{acc.x + hij * uj.x, acc.y + hij * uj.y};
-            }
-            v[k] = acc;
-        }
+        for (int k = 0; k < P; ++k) v[k] = eta.template ld<V>(vo, CV::soff(k));
         block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
         const Fld y = c.fld(i), st = c.fld(M + i), ac = c.fld(2 * M + i);
 #pragma unroll
@@ -266,22 +311,14 @@ __device__ __forceinline__ void mm_rhs(const MmCtx<R, N, T, M>& c, double zs, in
     const uint32_t vo = CV::voff();
     mm_rhs_head<R, N, T, M, TAPE>(c, zs, IN, U, HQ, tape_u, tape_h, tslot, v, lds);
     drain_memory();  // re-reads hq of step 3
-    // 4. per mode: eta_i = sum_j h_ij u_j, k_i = i cis(-D_i zs) shock ifft(eta_i)
+    // 4. eta_i = sum_j h_ij u_j (into the u fields), k_i = i cis(-D_i zs) shock ifft(eta_i)
+    mm_eta_inplace<R, N, T, M>(c, U, HQ);
+    drain_memory();  // re-reads eta
 #pragma unroll 1
     for (int i = 0; i < M; ++i) {
+        const Fld eta = c.fld(U + i);
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            V acc = V{(R)0, (R)0};
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                const int q = a.qidx[i * M + j];
-                const V hp = c.fld(HQ + (q >> 1)).template ld<V>(vo, CV::soff(k));
-                const R hij = (q & 1) ? hp.y : hp.x;
-                const V uj = c.fld(U + j).template ld<V>(vo, CV::soff(k));
-                acc = V{acc.x + hij * uj.x, acc.y + hij * uj.y};
-            }
-            v[k] = acc;
-        }
+        for (int k = 0; k < P; ++k) v[k] = eta.template ld<V>(vo, CV::soff(k));
         block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
         const Fld out = c.fld(OUT + i);
 #pragma unroll
@@ -416,18 +453,29 @@ __device__ __forceinline__ void mm_vjp(const MmCtx<R, N, T, M>& c, double zs, R 
             G[m] = c.fld(TR::GG + m).template ld<V>(vo, CV::soff(k));
             u[m] = tu(m).template ld<V>(vo, CV::soff(k));
         }
+        // gu_j = sum_i h_ij G_i with the stage's h pairs loaded once per sample
+        auto gu_from = [&](auto symc) {
+            constexpr bool SYM = decltype(symc)::value;
+            constexpr int QS = SYM ? (NP + 1) / 2 : QCMAX;
+            V hq[QS];
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            V g = V{(R)0, (R)0};
+            for (int cc = 0; cc < QS; ++cc) hq[cc] = th(cc).template ld<V>(vo, CV::soff(k));
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                const int q = a.qidx[i * M + j];
-                const V hp = th(q >> 1).template ld<V>(vo, CV::soff(k));
-                const R hij = (q & 1) ? hp.y : hp.x;
-                g = V{g.x + hij * G[i].x, g.y + hij * G[i].y};
+            for (int j = 0; j < M; ++j) {
+                V g = V{(R)0, (R)0};
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    const int q = pair_index(M, i, j, SYM);
+                    const R hij = (q & 1) ? hq[q >> 1].y : hq[q >> 1].x;
+                    g = V{g.x + hij * G[i].x, g.y + hij * G[i].y};
+                }
+                c.fld(TR::GU + j).st(g, vo, CV::soff(k));
             }
-            c.fld(TR::GU + j).st(g, vo, CV::soff(k));
-        }
+        };
+        if (sym)
+            gu_from(std::true_type{});
+        else
+            gu_from(std::false_type{});
         R A[QMAX + 1];
         A[QMAX] = (R)0;
         if (sym) {

@@ -348,6 +348,10 @@ template <typename R> int upload_tables(fh_plan p)
                 }
         }
         p->q = (int)outp.size();
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < M; ++j)
+                if (qidx[i * M + j] != fh::pair_index(M, i, j, sym))
+                    return fail(FH_EINVAL, "internal: output-pair order differs from the kernels' pair_index");
         const double tol = 1e-12 * std::max(1.0, gmax);
         auto close = [&](double x, double y) { return std::fabs(x - y) <= tol + 1e-12 * std::fabs(y); };
         for (int i = 0; i < M && p->gamma_full_sym; ++i)
