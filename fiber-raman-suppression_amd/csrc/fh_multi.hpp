@@ -92,6 +92,24 @@ __host__ __device__ constexpr int pair_index(int M, int i, int j, bool sym)
     return sym ? (i <= j ? i * M - i * (i - 1) / 2 + (j - i) : j * M - j * (j - 1) / 2 + (i - j)) : i * M + j;
 }
 
+// Input pairs p of the symmetric products Re(u_i conj u_j), i <= j in row
+// order (the column order of the reduced gamma Gr[q][p]), as a constant table
+// that folds to immediates when indexed from unrolled loops.
+template <int M> struct InputPairs {
+    static constexpr int NP = M * (M + 1) / 2;
+    int i[NP], j[NP];
+    constexpr InputPairs() : i(), j()
+    {
+        int p = 0;
+        for (int a = 0; a < M; ++a)
+            for (int b = a; b < M; ++b) {
+                i[p] = a;
+                j[p] = b;
+                ++p;
+            }
+    }
+};
+
 // Per-workgroup views shared by the stage and pullback routines.
 template <typename R, int N, int T, int M> struct MmCtx {
     using V = C2<R>;
@@ -101,6 +119,73 @@ template <typename R, int N, int T, int M> struct MmCtx {
     int Q, QC;
     __device__ __forceinline__ Fld fld(int f) const { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; }
 };
+
+typedef float fh_floatx16 __attribute__((ext_vector_type(16)));
+
+// Step 2 on the matrix cores (FP32, gamma symmetric in (i, j)): the Kerr
+// contraction dK_q(t) = sum_p Gr[q][p] b_p(t) is a [Q x P] x [P x N] product
+// computed with v_mfma_f32_32x32x2f32.  A wave takes 32 of its samples at a
+// time with two lanes per sample: lane l supplies the products b_p,
+// p = 2r + l/32, of sample l%32 (row l/32 of the 2 x 32 B tile) and the
+// reduced gamma Gr[qb 32 + l%32][2r + l/32] (the A tile, held in registers
+// for the whole call).  Accumulator r of lane l then holds output row
+// 8(r/4) + 4(l/32) + r%4 and column l%32, so each lane stores whole h pairs.
+// The samples of a chunk belong to lanes of the same wave; the caller's
+// drain_memory() orders their u stores before these loads.
+template <typename R, int N, int T, int M>
+__device__ __forceinline__ void mm_contract_mfma(const MmCtx<R, N, T, M>& c, int U, int HQ)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    constexpr int P = TR::P;
+    constexpr int NP = TR::NP;
+    constexpr int KS = (NP + 1) / 2;  // k-steps of two products
+    constexpr int QB = (NP + 31) / 32;  // 32-row blocks of output pairs (Q = NP)
+    static_assert(T % 64 == 0, "whole waves");
+    const int QC = c.QC;
+    const float* __restrict__ gr = c.a.gr;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, kh = l >> 5, j = l & 31;
+    float ar[QB][KS];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+        for (int r = 0; r < KS; ++r) {
+            const int q = qb * 32 + j, p = 2 * r + kh;
+            ar[qb][r] = (q < NP && p < NP) ? gr[q * NP + p] : 0.0f;
+        }
+#pragma unroll 1
+    for (int kk = 0; kk < P; ++kk) {
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t vs = (uint32_t)(w * 64 + 32 * h + j) * (uint32_t)sizeof(V);
+            V u[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) u[m] = c.fld(U + m).template ld<V>(vs, CV::soff(kk));
+            constexpr InputPairs<M> ip{};
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) {
+                fh_floatx16 acc = {0};
+#pragma unroll
+                for (int r = 0; r < KS; ++r) {
+                    // this lane's product: pair 2r + kh (zero past the last pair)
+                    const float ve = u[ip.i[2 * r]].x * u[ip.j[2 * r]].x + u[ip.i[2 * r]].y * u[ip.j[2 * r]].y;
+                    float vo = 0.0f;
+                    if (2 * r + 1 < NP)
+                        vo = u[ip.i[2 * r + 1]].x * u[ip.j[2 * r + 1]].x + u[ip.i[2 * r + 1]].y * u[ip.j[2 * r + 1]].y;
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[qb][r], kh ? vo : ve, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int ii = 0; ii < 4; ii += 2) {
+                        const int cc = (qb * 32 + 8 * g + 4 * kh + ii) >> 1;
+                        if (cc < QC) c.fld(HQ + cc).st(V{acc[4 * g + ii], acc[4 * g + ii + 1]}, vs, CV::soff(kk));
+                    }
+            }
+        }
+    }
+}
 
 // Steps 1-3 of the right-hand side at zs: time fields u_m = fft(cis(D_m zs)
 // y_m) of the input fields y_m (slice fields IN + m) into U + m, and the
@@ -145,7 +230,13 @@ __device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double z
         }
     }
     drain_memory();  // re-reads u of step 1
-    // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p
+    // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p (matrix cores
+    // for FP32 with symmetric gamma from M = 6 up, VALU otherwise; at most
+    // 512 threads leave room for the A tile of Gr in registers)
+    constexpr bool MFMA_OK = sizeof(R) == 4 && T <= 512 && M >= 6;
+    if (MFMA_OK && Q != M * M) {
+        if constexpr (MFMA_OK) mm_contract_mfma<R, N, T, M>(c, U, HQ);
+    } else {
 #pragma unroll 1
     for (int k = 0; k < P; ++k) {
         V uk[M];
@@ -172,6 +263,7 @@ __device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double z
             }
             c.fld(HQ + cc).st(V{d0, d1}, vo, CV::soff(k));
         }
+    }
     }
     drain_memory();  // re-reads hq of step 2
     // 3. Raman on packed pairs: h = (1-fR) dK + real(ifft(H fft(dK)))

@@ -64,6 +64,11 @@ def test_mode_counts_match_oracle(gpu_context, modes, nt, precision):
         assert rel(gpu, ref) <= 1e-11
         assert np.all(mode_energy_rel(gpu, ref) <= 1e-11)
     else:
+        # Kerr coupling mostly moves phase, so the field itself is checked as
+        # well: a dropped or misplaced contraction pair changes it by ~1e-2
+        # at this nonlinear phase (~0.04 rad), FP32 rounding by ~1e-7.
+        print("fp32 field rel", rel(gpu, ref), flush=True)
+        assert rel(gpu, ref) <= 1e-5
         assert np.all(mode_energy_rel(gpu, ref) <= 1e-5)
 
 
