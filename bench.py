@@ -122,12 +122,23 @@ def grin_launches(batch: int) -> np.ndarray:
     return np.stack(out)
 
 
-def time_gradient(problem, steps, precision, batch, k, warmup, mask):
+def kept_segments(nt, steps, every, precision, batch, budget_bytes):
+    """Tape segments the gradient plan keeps within a device-memory budget:
+    each kept segment holds 4*every stage slots of u (complex) and h (real)
+    per solve, and one more region serves the recomputed segments."""
+    esz = 8 if precision == "fp64" else 4
+    per_segment = 4 * every * nt * 3 * esz * batch
+    nseg = -(-steps // every)
+    k = int(budget_bytes // per_segment) - 1
+    return max(0, min(nseg, k))
+
+
+def time_gradient(problem, steps, precision, batch, k, warmup, mask, tape_segments=0):
     """K event-timed launches of the fused gradient kernel (band objective on device)."""
     import fiberhip
 
     plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch, band_mask=mask,
-                                      checkpoint_every=50)
+                                      checkpoint_every=50, tape_segments=tape_segments)
     u0 = plan.to_device(launches(problem, batch, seed=2)[:, :, None])
     out = fiberhip.DeviceArray(plan.context, u0.nbytes)
     lam0 = fiberhip.DeviceArray(plan.context, u0.nbytes)
@@ -338,6 +349,9 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=3, help="timed launches per path")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256, help="gradients per launch (SMF-28 2 m gradient workload)")
+    ap.add_argument("--tape-budget-gib", type=float, default=200.0,
+                    help="device memory the gradient plans may spend on kept stage tape (same budget for FP32 "
+                         "and FP64; 0 recomputes every segment)")
     ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (SMF-28 0.5 m forward workload)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "serving"], default="all")
@@ -371,7 +385,10 @@ def main() -> None:
         problem, steps = workload("smf28-grad-16384")
         nt, m = problem.nt, problem.modes
         mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
-        g = {p: time_gradient(problem, steps, p, args.batch, args.steps, args.warmup, mask) for p in ("fp32", "fp64")}
+        budget = int(args.tape_budget_gib * (1 << 30))
+        kept = {p: kept_segments(nt, steps, 50, p, args.batch, budget) for p in ("fp32", "fp64")}
+        g = {p: time_gradient(problem, steps, p, args.batch, args.steps, args.warmup, mask, kept[p])
+             for p in ("fp32", "fp64")}
         ms = g["fp32"]["kernel_ms"]
         value = args.batch / (ms / 1e3)
         fp64_value = args.batch / (g["fp64"]["kernel_ms"] / 1e3)
@@ -395,7 +412,8 @@ def main() -> None:
             "config": {"workload": f"SMF-28 2 m, 0.2 W, 185 fs sech2, Nt={nt}, T={problem.sim['time_window']} ps, "
                                    f"{steps} RK4IP steps, checkpoint every 50, band -16..-10 THz objective, "
                                    f"gradient w.r.t. the full-grid spectral phase",
-                       "batch_gradients_per_step": args.batch, "modes": m, "nt": nt},
+                       "batch_gradients_per_step": args.batch, "modes": m, "nt": nt,
+                       "kept_tape_segments": kept, "tape_budget_gib": args.tape_budget_gib},
             "fp64_value": fp64_value,
             "fp32_over_fp64": value / fp64_value,
             "fp32_vs_fp64": {"max_rel_output_spectral_energy_err": float(np.max(np.abs(e32 - e64) / e64)),
@@ -410,6 +428,7 @@ def main() -> None:
             "fp64_roofline": {"achieved": args.batch * flops / (g["fp64"]["kernel_ms"] / 1e3) / 1e12,
                               "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "kernel": "k_grad_single<double,16384,1024>"},
             "device_bytes_fp32_plan": g["fp32"]["device_bytes"],
+            "device_bytes_fp64_plan": g["fp64"]["device_bytes"],
         }
         if not args.no_cpu:
             cb = cpu_baseline_gradient(problem, steps, mask)

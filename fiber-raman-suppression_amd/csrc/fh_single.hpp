@@ -50,9 +50,10 @@ template <typename R> struct SmArgs {
     double* grad;               // [batch][N] 2 Re(conj(lam0) i u0)
     double* cost;               // [batch]
     C2<R>* ckpt;                // [batch][nseg][N]
-    C2<R>* tape_u;              // [batch][4*every][N]
-    R* tape_h;                  // [batch][4*every][N]
+    C2<R>* tape_u;              // [batch][regions][4*every][N]
+    R* tape_h;                  // [batch][regions][4*every][N]
     int every, nseg;
+    int ntaped;  // the last ntaped segments keep the forward sweep's tape (k_grad_single)
 };
 
 template <typename R, int N, int T> struct SmTraits;
@@ -347,6 +348,13 @@ template <int T> __device__ __forceinline__ double block_sum(double x, void* lds
 // backward sweep of the discrete adjoint with per-segment recomputation, and
 // the phase chain rule g = 2 Re(conj(lam0) i u0) (physics_models.jl:1042) in
 // FP64.
+//
+// The tape of a solve has one region of 4*every stage slots per kept segment
+// (the last ntaped segments, written by the forward sweep) plus one shared
+// region for the segments the backward sweep recomputes.  The recomputation
+// repeats the forward sweep's arithmetic from the checkpoint, so a kept
+// segment's tape holds the same values and the result does not depend on
+// ntaped.
 template <typename R, int N, int T>
 __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
 {
@@ -370,8 +378,13 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
     // Fixed descriptors per solve region; checkpoints and tape slots are
     // addressed through the per-lane offset.
     const Buf ckpt(a.ckpt + sol * (size_t)a.nseg * N, (uint32_t)((size_t)a.nseg * N * sizeof(V)));
-    const Buf tape_u(a.tape_u + sol * (size_t)4 * a.every * N, (uint32_t)((size_t)4 * a.every * N * sizeof(V)));
-    const Buf tape_h(a.tape_h + sol * (size_t)4 * a.every * N, (uint32_t)((size_t)4 * a.every * N * sizeof(R)));
+    const int first_kept = a.nseg - a.ntaped;  // segments first_kept.. keep their tape
+    const int seg_slots = 4 * a.every;
+    const size_t tape_slots = (size_t)(a.ntaped + (first_kept > 0 ? 1 : 0)) * seg_slots;
+    const Buf tape_u(a.tape_u + sol * tape_slots * N, (uint32_t)(tape_slots * N * sizeof(V)));
+    const Buf tape_h(a.tape_h + sol * tape_slots * N, (uint32_t)(tape_slots * N * sizeof(R)));
+    // first tape slot of segment sg: its own region, or the shared one
+    auto seg_base = [&](int sg) { return (sg >= first_kept ? sg - first_kept : a.ntaped) * seg_slots; };
     const Buf none(a.u0, 0);
     const double h = a.h;
 
@@ -382,13 +395,19 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
         v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
         st.set(k, v[k]);
     }
-    for (int step = 0; step < a.steps; ++step) {
+    const int kept_from = min(first_kept * a.every, a.steps);
+    for (int step = 0; step < kept_from; ++step) {
         if (step % a.every == 0) {
             const Fld ck{ckpt, (uint32_t)(step / a.every) * (uint32_t)(N * sizeof(V))};
 #pragma unroll
             for (int k = 0; k < P; ++k) ck.st(v[k], CV::voff(), CV::soff(k));
         }
         sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, none, none, 0, lds);
+    }
+    for (int step = kept_from; step < a.steps; ++step) {
+        const int sg = step / a.every;
+        sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, seg_base(sg) + 4 * (step - sg * a.every),
+                               lds);
     }
     // output and terminal seed
     const Buf out(a.out + sol * N, N * sizeof(V));
@@ -435,14 +454,17 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
     for (int sg = a.nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * a.every;
         const int s1 = min(s0 + a.every, a.steps);
-        const Fld ck{ckpt, (uint32_t)sg * (uint32_t)(N * sizeof(V))};
+        const int base = seg_base(sg);
+        if (sg < first_kept) {  // recompute the segment's tape from its checkpoint
+            const Fld ck{ckpt, (uint32_t)sg * (uint32_t)(N * sizeof(V))};
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            v[k] = ck.template ld<V>(CV::voff(), CV::soff(k));
-            st.set(k, v[k]);
+            for (int k = 0; k < P; ++k) {
+                v[k] = ck.template ld<V>(CV::voff(), CV::soff(k));
+                st.set(k, v[k]);
+            }
+            for (int step = s0; step < s1; ++step)
+                sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, base + 4 * (step - s0), lds);
         }
-        for (int step = s0; step < s1; ++step)
-            sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, 4 * (step - s0), lds);
         for (int step = s1 - 1; step >= s0; --step) {
             const double z = step * h;
 #pragma unroll 1
@@ -457,7 +479,7 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
                     if (s == 3) lacc.set(k, l);
                     v[k] = V{cl * l.x + cg * v[k].x, cl * l.y + cg * v[k].y};
                 }
-                const int slot = 4 * (step - s0) + s;
+                const int slot = base + 4 * (step - s0) + s;
                 const Fld tu{tape_u, (uint32_t)slot * (uint32_t)(N * sizeof(V))};
                 const Fld th{tape_h, (uint32_t)slot * (uint32_t)(N * sizeof(R))};
                 sm_vjp<R, N, T>(v, zs, a, tb, gu, tu, th, lds);

@@ -49,6 +49,7 @@ struct fh_plan_s {
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
     int every = 50, nseg = 0;
+    int ntaped = 0;  // final segments whose stage tape the forward sweep keeps
     bool has_mask = false;
     DevBuf gr, qidx;  // multimode: reduced gamma [Q][P] and output-pair index [M][M]
     int q = 0;
@@ -254,6 +255,9 @@ int check_desc(const fh_problem_desc* d)
     if (dmax * d->length_m >= FH_MAX_PHASE)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
+    if (d->tape_segments < 0) return fail(FH_EINVAL, "tape_segments must be >= 0");
+    if (d->tape_segments > 0 && (d->modes != 1 || d->scheme != FH_SCHEME_RK4IP_FIXED))
+        return fail(FH_EINVAL, "tape_segments applies to the fixed-step single-mode gradient");
     if (d->modes == 1) {
         const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
         if (d->nt < 64 || d->nt > nmax)
@@ -311,6 +315,7 @@ template <typename R> int upload_tables(fh_plan p)
     } else {
         if (p->every > d.steps) p->every = d.steps;
         p->nseg = (d.steps + p->every - 1) / p->every;
+        p->ntaped = std::min(d.tape_segments, p->nseg);
         p->work_per_solve = d.modes == 1 ? with_size<R, OpWork>(nt) : (size_t)fh::multi_fields(d.modes) * nt;
     }
     if (d.modes > 1) {
@@ -419,6 +424,7 @@ template <typename R> fh::SmArgs<R> base_args(fh_plan p)
     a.steps = d.steps;
     a.every = p->every;
     a.nseg = p->nseg;
+    a.ntaped = p->ntaped;
     a.log_cost = d.log_cost;
     a.mask = (const unsigned char*)p->mask.p;
     return a;
@@ -438,8 +444,14 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
     const size_t stages = p->adaptive ? 6 : 4;
     // tape per stage: u of every mode; h as one real field (single mode) or ceil(Q/2) packed pairs
     const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((p->q + 1) / 2);
-    if ((rc = dev_alloc(p, p->tape_u, V * nt * M * stages * (size_t)p->every * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_h, hbytes * stages * (size_t)p->every * mb))) return rc;
+    // fixed-step single mode: one region per kept segment plus the shared
+    // recomputation region when some segments are recomputed
+    const size_t regions = (size_t)p->ntaped + (p->ntaped < p->nseg ? 1 : 0);
+    const size_t solve_tape = V * nt * M * stages * (size_t)p->every * regions;
+    if (solve_tape > 0xffffffffull)
+        return fail(FH_EINVAL, "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every");
+    if ((rc = dev_alloc(p, p->tape_u, solve_tape * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, hbytes * stages * (size_t)p->every * regions * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_lam0, V * nt * M * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;

@@ -68,7 +68,7 @@ class ProblemDesc(ctypes.Structure):
         ("rtol", ctypes.c_double),
         ("atol", ctypes.c_double),
         ("max_steps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("tape_segments", ctypes.c_int32),
     ]
 
 

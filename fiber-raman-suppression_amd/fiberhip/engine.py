@@ -101,7 +101,7 @@ class Plan:
     def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: Optional[int],
                  precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None,
                  band_mask=None, checkpoint_every: int = 50, log_cost: bool = False, scheme: Optional[str] = None,
-                 rtol: float = 1e-8, atol: float = 1e-6, max_steps: int = 10000):
+                 rtol: float = 1e-8, atol: float = 1e-6, max_steps: int = 10000, tape_segments: int = 0):
         self.context = context or default_context()
         D = np.asarray(dispersion, dtype=np.float64)
         if D.ndim == 1:
@@ -132,7 +132,7 @@ class Plan:
             shock=_dp(self._sh), one_m_fr=float(one_m_fr), checkpoint_every=int(checkpoint_every),
             log_cost=int(bool(log_cost)),
             band_mask=None if self._mask is None else self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-            rtol=self.rtol, atol=self.atol, max_steps=self.max_steps)
+            rtol=self.rtol, atol=self.atol, max_steps=self.max_steps, tape_segments=int(tape_segments))
         h = ctypes.c_void_p()
         check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
@@ -141,7 +141,7 @@ class Plan:
     def from_problem(cls, problem, steps: Optional[int] = None, precision: str = "fp64", max_batch: int = 1,
                      context: Optional[Context] = None, band_mask=None, checkpoint_every: int = 50,
                      log_cost: bool = False, scheme: Optional[str] = None, rtol: Optional[float] = None,
-                     atol: Optional[float] = None, max_steps: int = 10000) -> "Plan":
+                     atol: Optional[float] = None, max_steps: int = 10000, tape_segments: int = 0) -> "Plan":
         """Plan for a FiberFieldProblem.  With `steps` the fixed-step RK4IP of
         _scalar_rk4_output; without, the adaptive Tsit5 of solve_disp_mmf with the
         fiber's reltol/abstol (defaults 1e-8 / 1e-6, simulate_disp_mmf.jl:186-190)."""
@@ -153,7 +153,7 @@ class Plan:
         return cls(f["Dω"], f["γ"], f["hRω"], self_steepening(problem.sim), f["one_m_fR"], f["L"], steps,
                    precision=precision, max_batch=max_batch, context=context, band_mask=band_mask,
                    checkpoint_every=checkpoint_every, log_cost=log_cost, scheme=scheme, rtol=rtol, atol=atol,
-                   max_steps=max_steps)
+                   max_steps=max_steps, tape_segments=tape_segments)
 
     def solve_stats(self, batch: int) -> np.ndarray:
         """(batch, 3) accepted, rejected, rhs evaluations of the last call (fh_solve_stats)."""

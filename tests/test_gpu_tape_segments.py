@@ -1,0 +1,102 @@
+"""Kept tape segments of the fixed-step single-mode gradient (tape_segments).
+
+The forward sweep stores the stage tape of the last K checkpoint segments and
+the backward sweep recomputes only the others.  The recomputation repeats
+the forward sweep's arithmetic from the checkpoint, so every output (uL,
+cost, lam0, grad) must be bit-identical for every K, in both precisions,
+including a partial last segment and K at or beyond the segment count.  The
+K = 0 path is the one pinned against the oracle's discrete adjoint in
+tests/test_gpu_adjoint.py; one kept-tape case is also checked against the
+oracle directly here.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import oracle_fiber, rel, smf_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _launches(problem, n, seed):
+    import fiberhip
+
+    rng = np.random.default_rng(seed)
+    basis = fiberhip.taylor_phase_basis(problem, (2, 3))
+    phases = [basis @ np.array([rng.uniform(-1e4, 1e4), rng.uniform(-5e5, 5e5)]) for _ in range(n)]
+    return np.stack([problem.uω0 * np.exp(1j * p)[:, None] for p in phases])
+
+
+def _gradients(problem, mask, launches, steps, every, precision, tapes):
+    import fiberhip
+
+    out = {}
+    for k in tapes:
+        print(f"case: tape_segments={k} nt={problem.sim['Nt']} steps={steps} every={every} {precision}", flush=True)
+        plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, band_mask=mask,
+                                          checkpoint_every=every, max_batch=len(launches), tape_segments=k)
+        out[k] = plan.gradient(launches)
+        del plan
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_kept_tape_is_bit_identical(gpu_context, precision):
+    import fiberhip
+
+    problem = smf_problem(nt=1024, window_ps=10.0, length_m=0.5, power_w=0.2)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    launches = _launches(problem, 3, 5)
+    # 60 steps in segments of 7: 9 segments, the last one of 4 steps
+    res = _gradients(problem, mask, launches, 60, 7, precision, (0, 1, 4, 9, 20))
+    for k in (1, 4, 9, 20):
+        for key in ("uL", "cost", "lam0", "grad"):
+            assert np.array_equal(res[k][key], res[0][key]), (precision, k, key)
+
+
+def test_kept_tape_fp64_matches_oracle(gpu_context):
+    import fiberhip
+    import oracle
+    from oracle import np_ref as R
+
+    problem = smf_problem(nt=1024, window_ps=10.0, length_m=0.5, power_w=0.2)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    launches = _launches(problem, 1, 9)
+    steps, every = 60, 7
+    res = _gradients(problem, mask, launches, steps, every, "fp64", (3,))[3]
+    fib = oracle_fiber(problem)
+    uL = oracle.rk4_forward(fib, problem.fiber["L"], steps, launches[0])
+    J, seed = R.band_cost(uL, mask)
+    lam0 = oracle.rk4_adjoint(fib, problem.fiber["L"], steps, launches[0], seed, every=every)
+    assert rel(res["uL"][0], uL) <= 1e-11
+    assert abs(res["cost"][0] - J) <= 1e-12 * J
+    assert rel(res["lam0"][0], lam0) <= 1e-10
+    assert rel(res["grad"][0], R.phase_gradient(lam0, launches[0])) <= 1e-10
+
+
+def test_kept_tape_headline_kernel_bit_identical(gpu_context):
+    """The 16384-sample FP32 kernel of the SMF-28 2 m gradient (1024 threads,
+    state in global scratch), 100 steps in 4 segments, 2 of them kept."""
+    import fiberhip
+
+    problem = smf_problem(nt=16384, window_ps=47.061, length_m=0.1, power_w=0.1)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    launches = _launches(problem, 2, 3)
+    res = _gradients(problem, mask, launches, 100, 25, "fp32", (0, 2))
+    for key in ("uL", "cost", "lam0", "grad"):
+        assert np.array_equal(res[2][key], res[0][key]), key
+
+
+def test_tape_segments_validation(gpu_context):
+    import fiberhip
+    from fiberhip import grin
+
+    problem = smf_problem(nt=1024, window_ps=10.0, length_m=0.5, power_w=0.2)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
+        fiberhip.Plan.from_problem(problem, steps=60, tape_segments=-1)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
+        fiberhip.Plan.from_problem(problem, scheme="tsit5", tape_segments=2)
+    gp = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.002, energy_j=1e-9, nx=96, window_um=60.0)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
+        fiberhip.Plan.from_problem(gp, steps=10, tape_segments=1)

@@ -107,10 +107,11 @@ def test_library_loads_and_exports_header_symbols():
 
 def test_struct_layout_matches_header():
     """ProblemDesc: 6 int32, length_m, 4 pointers, one_m_fr, 2 int32, band_mask
-    pointer, rtol, atol, max_steps, reserved."""
+    pointer, rtol, atol, max_steps, tape_segments."""
     assert ctypes.sizeof(_lib.ProblemDesc) == 6 * 4 + 8 + 4 * 8 + 8 + 2 * 4 + 8 + 8 + 8 + 2 * 4
     assert _lib.ProblemDesc.band_mask.offset == 80
     assert _lib.ProblemDesc.rtol.offset == 88 and _lib.ProblemDesc.max_steps.offset == 104
+    assert _lib.ProblemDesc.tape_segments.offset == 108
     assert _lib.ProblemDesc.length_m.offset == 24
     assert ctypes.sizeof(_lib.Stats) == 32
     assert ctypes.sizeof(_lib.ServerConfig) == 8 + 4 * 4 + 3 * 8 + 2 * 4
