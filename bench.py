@@ -154,7 +154,7 @@ def time_gradient(problem, steps, precision, batch, k, warmup, mask, tape_segmen
                 device_bytes=plan.device_bytes)
 
 
-def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol=1e-6, atol=1e-6):
+def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol=1e-6, atol=1e-6, tape_segments=0):
     """Adaptive candidate sweep: `candidates` forward + adjoint gradients of the band
     objective, launched `per_launch` at a time back to back on one stream; each
     candidate runs its own Tsit5 step sequence.  Returns the per-sweep time and
@@ -162,7 +162,8 @@ def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol
     import fiberhip
 
     plan = fiberhip.Plan.from_problem(problem, scheme="tsit5", rtol=rtol, atol=atol, precision=precision,
-                                      max_batch=per_launch, band_mask=mask, checkpoint_every=50, max_steps=4000)
+                                      max_batch=per_launch, band_mask=mask, checkpoint_every=50, max_steps=4000,
+                                      tape_segments=tape_segments)
     host = launches(problem, candidates, seed=4)[:, :, None]
     nl = candidates // per_launch
     u0 = [plan.to_device(host[i * per_launch:(i + 1) * per_launch]) for i in range(nl)]
@@ -356,6 +357,9 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "serving"], default="all")
     ap.add_argument("--sweep-candidates", type=int, default=256, help="adaptive candidate sweep size")
+    ap.add_argument("--sweep-tape-segments", type=int, default=3,
+                    help="accepted-step segments of 50 whose stage tape the sweep's forward pass keeps (0 recomputes "
+                         "every segment)")
     ap.add_argument("--serve-requests", type=int, default=100, help="requests in the serving trace")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
@@ -498,10 +502,11 @@ def main() -> None:
         tune = {}
         for per in (nc, nc // 2, nc // 4):
             if per >= 1 and nc % per == 0:
-                tune[per] = time_sweep(problem, "fp32", nc, per, 1, 1, mask)["ms"]
+                tune[per] = time_sweep(problem, "fp32", nc, per, 1, 1, mask, tape_segments=args.sweep_tape_segments)["ms"]
                 print(f"bench: sweep schedule {nc // per} x {per}: {tune[per]:.1f} ms", file=sys.stderr, flush=True)
         per = min(tune, key=tune.get)
-        res = {p: time_sweep(problem, p, nc, per, args.steps, args.warmup, mask) for p in ("fp32", "fp64")}
+        res = {p: time_sweep(problem, p, nc, per, args.steps, args.warmup, mask, tape_segments=args.sweep_tape_segments)
+               for p in ("fp32", "fp64")}
         st = res["fp64"]["stats"]
         # algorithmic flops from the actual evaluation counts: forward RHS (accepted +
         # rejected + starting step) and 6 stage pullbacks per accepted step
@@ -512,6 +517,7 @@ def main() -> None:
                         f"GDD/TOD/FOD candidates, adaptive Tsit5 rtol=atol=1e-6, forward + adjoint gradient of the "
                         f"-16..-10 THz band objective",
             "candidates": nc, "schedule": f"{nc // per} launches x {per} candidates", "autotune_ms": tune,
+            "kept_tape_segments": args.sweep_tape_segments,
             "fp32_grads_per_s": nc / (ms32 / 1e3), "fp64_grads_per_s": nc / (ms64 / 1e3),
             "fp32_over_fp64": ms64 / ms32,
             "fp64_accepted_steps_mean": float(np.mean(st[:, 0])), "fp64_rejected_steps_mean": float(np.mean(st[:, 1])),

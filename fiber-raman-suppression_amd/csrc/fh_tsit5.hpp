@@ -134,11 +134,16 @@ __device__ __forceinline__ double scaled_sq(V e, V u, V w, double s, double rtol
 // workgroup-uniform status is 0 (ok), 1 (non-finite) or 2 (max_steps accepted
 // or max_steps rejected steps).
 // With CKPT, the state and position at accepted steps 0, every, 2 every, ...
-// are stored to ckpt.
+// are stored to ckpt, and every stage evaluation stores its u and h to the
+// stage tape: slot 6 (n mod every) + i of accepted-step index n, in the
+// region of segment n / every when that segment is one of the first ntaped
+// (kept), else in the shared region the backward sweep recomputes into.  An
+// attempt that is rejected is overwritten by the next one; the seventh stage
+// (first same as last) is stage 0 of step n + 1.
 template <typename R, int N, int T, bool CKPT, bool SAVE = false>
 __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fld& slot0,
-                          typename SmCtx<R, N, T>::FU& ut, const Buf& ckpt, int* ys_out, long long* nacc_out,
-                          long long* nrej_out, long long* nfe_out, void* lds)
+                          typename SmCtx<R, N, T>::FU& ut, const Buf& ckpt, const Buf& tape_u, const Buf& tape_h,
+                          int* ys_out, long long* nacc_out, long long* nrej_out, long long* nfe_out, void* lds)
 {
     using V = C2<R>;
     using CV = Cyc<V, T>;
@@ -156,6 +161,13 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     // k0 and k6 trade slots on acceptance (first same as last); k1..k5 are fixed
     int sk0 = LY::K0, sk6 = LY::K0 + 6;
     auto kslot = [&](int j) { return j == 0 ? sk0 : (j == 6 ? sk6 : LY::K0 + j); };
+    auto tape_slot = [&](long long n, int i) {
+        const int sg = (int)(n / a.s.every);
+        const int region = sg < a.s.ntaped ? sg : a.s.ntaped;
+        return (uint32_t)(region * 6 * a.s.every + 6 * (int)(n % a.s.every) + i);
+    };
+    auto tu_at = [&](uint32_t slot) { return CKPT ? Fld{tape_u, slot * (uint32_t)(N * sizeof(V))} : nf; };
+    auto th_at = [&](uint32_t slot) { return CKPT ? Fld{tape_h, slot * (uint32_t)(N * sizeof(R))} : nf; };
     V v[P];
     long long nfe = 0, nacc = 0, nrej = 0;
 
@@ -165,7 +177,7 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
 #pragma unroll
         for (int k = 0; k < P; ++k) v[k] = y.template ld<V>(vo, CV::soff(k));
     }
-    sm_rhs<R, N, T, false>(v, 0.0, a.s, tb, ut, nf, nf, lds);
+    sm_rhs<R, N, T, CKPT>(v, 0.0, a.s, tb, ut, tu_at(tape_slot(0, 0)), th_at(tape_slot(0, 0)), lds);
     nfe++;
     double s0 = 0, s1 = 0;
     {
@@ -267,7 +279,8 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
 #pragma unroll
                 for (int k = 0; k < P; ++k) yn.st(v[k], vo, CV::soff(k));
             }
-            sm_rhs<R, N, T, false>(v, z + TB::c[i] * h, a.s, tb, ut, nf, nf, lds);
+            const uint32_t ts = i < 6 ? tape_slot(nacc, i) : tape_slot(nacc + 1, 0);
+            sm_rhs<R, N, T, CKPT>(v, z + TB::c[i] * h, a.s, tb, ut, tu_at(ts), th_at(ts), lds);
             const Fld ki = fld(kslot(i));
 #pragma unroll
             for (int k = 0; k < P; ++k) ki.st(v[k], vo, CV::soff(k));
@@ -361,7 +374,8 @@ __global__ __launch_bounds__(T) void k_tsit5_single(TsArgs<R> a)
     int ys;
     long long nacc, nrej, nfe;
     const Buf none(a.s.u0, 0);
-    const int status = ts_forward<R, N, T, false, SAVE>(a, tb, base, ut, none, &ys, &nacc, &nrej, &nfe, lds);
+    const int status = ts_forward<R, N, T, false, SAVE>(a, tb, base, ut, none, none, none, &ys, &nacc, &nrej, &nfe,
+                                                        lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
@@ -408,8 +422,11 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
     typename CX::FU ut(Buf(slice + (size_t)LY::CORE_GRAD * N, N * sizeof(V)));
     typename CX::FU gu(Buf(slice + (size_t)(LY::CORE_GRAD + 1) * N, N * sizeof(V)));
     const Buf ckpt(sa.ckpt + sol * (size_t)sa.nseg * N, (uint32_t)((size_t)sa.nseg * N * sizeof(V)));
-    const Buf tape_u(sa.tape_u + sol * (size_t)6 * sa.every * N, (uint32_t)((size_t)6 * sa.every * N * sizeof(V)));
-    const Buf tape_h(sa.tape_h + sol * (size_t)6 * sa.every * N, (uint32_t)((size_t)6 * sa.every * N * sizeof(R)));
+    // stage tape: one region of 6*every slots per kept segment (the first
+    // ntaped) plus the shared region of the recomputed segments
+    const size_t tslots = (size_t)(sa.ntaped + 1) * 6 * sa.every;
+    const Buf tape_u(sa.tape_u + sol * tslots * N, (uint32_t)(tslots * N * sizeof(V)));
+    const Buf tape_h(sa.tape_h + sol * tslots * N, (uint32_t)(tslots * N * sizeof(R)));
     const Buf u0(sa.u0 + sol * N, N * sizeof(V));
     const uint32_t vo = CV::voff();
     {
@@ -420,7 +437,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
     drain_memory();
     int ys;
     long long nacc, nrej, nfe;
-    int bad = ts_forward<R, N, T, true>(a, tb, base, ut, ckpt, &ys, &nacc, &nrej, &nfe, lds);
+    int bad = ts_forward<R, N, T, true>(a, tb, base, ut, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
@@ -483,15 +500,17 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * sa.every;
         const int s1 = (long long)s0 + sa.every < nacc ? s0 + sa.every : (int)nacc;
+        const bool kept = sg < sa.ntaped;  // the forward sweep left this segment's tape
+        const uint32_t rbase = (uint32_t)((kept ? sg : sa.ntaped) * 6 * sa.every);
         // recompute the segment with stage tapes; y lives in slot Y
-        {
+        if (!kept) {
             const Fld ck{ckpt, (uint32_t)sg * FB};
             const Fld y = fld(LY::Y);
             drain_memory();
 #pragma unroll
             for (int k = 0; k < P; ++k) y.st(ck.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
         }
-        for (int s = s0; s < s1; ++s) {
+        for (int s = s0; s < (kept ? s0 : s1); ++s) {
             const double h = a.hs[sol * a.max_steps + s], z = a.zs[sol * a.max_steps + s];
             const R hr = (R)h;
 #pragma unroll 1
@@ -512,7 +531,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
                     const V u = y.template ld<V>(vo, CV::soff(k));
                     v[k] = V{u.x + hr * acc.x, u.y + hr * acc.y};
                 }
-                const uint32_t slot = (uint32_t)(6 * (s - s0) + i);
+                const uint32_t slot = rbase + (uint32_t)(6 * (s - s0) + i);
                 const Fld tu{tape_u, slot * (uint32_t)(N * sizeof(V))};
                 const Fld th{tape_h, slot * (uint32_t)(N * sizeof(R))};
                 sm_rhs<R, N, T, true>(v, z + TB::c[i] * h, sa, tb, ut, tu, th, lds);
@@ -565,7 +584,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
                     }
                     v[k] = g;
                 }
-                const uint32_t slot = (uint32_t)(6 * (s - s0) + i);
+                const uint32_t slot = rbase + (uint32_t)(6 * (s - s0) + i);
                 const Fld tu{tape_u, slot * (uint32_t)(N * sizeof(V))};
                 const Fld th{tape_h, slot * (uint32_t)(N * sizeof(R))};
                 sm_vjp<R, N, T>(v, z + TB::c[i] * h, sa, tb, gu, tu, th, lds);

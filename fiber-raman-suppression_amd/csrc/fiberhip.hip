@@ -49,7 +49,7 @@ struct fh_plan_s {
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
     int every = 50, nseg = 0;
-    int ntaped = 0;  // final segments whose stage tape the forward sweep keeps
+    int ntaped = 0;  // segments whose stage tape the forward sweep keeps (single mode)
     bool has_mask = false;
     DevBuf gr, qidx;  // multimode: reduced gamma [Q][P] and output-pair index [M][M]
     int q = 0;
@@ -256,8 +256,8 @@ int check_desc(const fh_problem_desc* d)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->tape_segments < 0) return fail(FH_EINVAL, "tape_segments must be >= 0");
-    if (d->tape_segments > 0 && (d->modes != 1 || d->scheme != FH_SCHEME_RK4IP_FIXED))
-        return fail(FH_EINVAL, "tape_segments applies to the fixed-step single-mode gradient");
+    if (d->tape_segments > 0 && d->modes != 1)
+        return fail(FH_EINVAL, "tape_segments applies to the single-mode gradients");
     if (d->modes == 1) {
         const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
         if (d->nt < 64 || d->nt > nmax)
@@ -310,6 +310,7 @@ template <typename R> int upload_tables(fh_plan p)
         p->max_steps = d.max_steps > 0 ? d.max_steps : 10000;
         if (p->every > p->max_steps) p->every = p->max_steps;
         p->nseg = (p->max_steps + p->every - 1) / p->every;
+        p->ntaped = std::min(d.tape_segments, p->nseg);
         p->work_per_solve = d.modes == 1 ? (size_t)fh::tsit5_fields(nt, sizeof(R) == 8) * nt
                                          : (size_t)fh::tsit5_multi_fields(d.modes) * nt;
     } else {
@@ -444,9 +445,10 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
     const size_t stages = p->adaptive ? 6 : 4;
     // tape per stage: u of every mode; h as one real field (single mode) or ceil(Q/2) packed pairs
     const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((p->q + 1) / 2);
-    // fixed-step single mode: one region per kept segment plus the shared
-    // recomputation region when some segments are recomputed
-    const size_t regions = (size_t)p->ntaped + (p->ntaped < p->nseg ? 1 : 0);
+    // single mode: one region per kept segment plus the shared recomputation
+    // region (fixed step: only when some segments are recomputed; adaptive:
+    // always, since the segment count is known only at the end of a solve)
+    const size_t regions = (size_t)p->ntaped + ((p->adaptive || p->ntaped < p->nseg) ? 1 : 0);
     const size_t solve_tape = V * nt * M * stages * (size_t)p->every * regions;
     if (solve_tape > 0xffffffffull)
         return fail(FH_EINVAL, "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every");

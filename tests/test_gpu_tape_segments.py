@@ -1,7 +1,8 @@
-"""Kept tape segments of the fixed-step single-mode gradient (tape_segments).
+"""Kept tape segments of the single-mode gradients (tape_segments).
 
-The forward sweep stores the stage tape of the last K checkpoint segments and
-the backward sweep recomputes only the others.  The recomputation repeats
+The forward sweep stores the stage tape of K checkpoint segments (the last K
+of a fixed-step solve, the first K of an adaptive one) and the backward sweep
+recomputes only the others.  The recomputation repeats
 the forward sweep's arithmetic from the checkpoint, so every output (uL,
 cost, lam0, grad) must be bit-identical for every K, in both precisions,
 including a partial last segment and K at or beyond the segment count.  The
@@ -88,6 +89,33 @@ def test_kept_tape_headline_kernel_bit_identical(gpu_context):
         assert np.array_equal(res[2][key], res[0][key]), key
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adaptive_kept_tape_is_bit_identical(gpu_context, precision):
+    """Adaptive single-mode gradient: the first K accepted-step segments keep
+    the forward sweep's stage tape (rejected attempts overwrite their slots;
+    the seventh stage is stage 0 of the next step).  Launches of different
+    chirps take different step counts; segments of 7 accepted steps."""
+    import fiberhip
+
+    problem = smf_problem(nt=1024, window_ps=10.0, length_m=1.0, power_w=0.3)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    launches = _launches(problem, 3, 21)
+    res = {}
+    for k in (0, 1, 3, 400):
+        print(f"case: adaptive tape_segments={k} nt=1024 every=7 {precision}", flush=True)
+        plan = fiberhip.Plan.from_problem(problem, scheme="tsit5", rtol=1e-7, atol=1e-6, precision=precision,
+                                          band_mask=mask, checkpoint_every=7, max_steps=2000, max_batch=3,
+                                          tape_segments=k)
+        res[k] = plan.gradient(launches)
+        steps = plan.solve_stats(3)[:, 0]
+        del plan
+    print("accepted steps", steps, flush=True)
+    assert steps.min() > 14  # more than two segments of 7 steps in every solve
+    for k in (1, 3, 400):
+        for key in ("uL", "cost", "lam0", "grad"):
+            assert np.array_equal(res[k][key], res[0][key]), (precision, k, key)
+
+
 def test_tape_segments_validation(gpu_context):
     import fiberhip
     from fiberhip import grin
@@ -95,8 +123,8 @@ def test_tape_segments_validation(gpu_context):
     problem = smf_problem(nt=1024, window_ps=10.0, length_m=0.5, power_w=0.2)
     with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
         fiberhip.Plan.from_problem(problem, steps=60, tape_segments=-1)
-    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
-        fiberhip.Plan.from_problem(problem, scheme="tsit5", tape_segments=2)
     gp = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.002, energy_j=1e-9, nx=96, window_um=60.0)
     with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
         fiberhip.Plan.from_problem(gp, steps=10, tape_segments=1)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
+        fiberhip.Plan.from_problem(gp, scheme="tsit5", tape_segments=1)
