@@ -192,6 +192,7 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
     d.rtol = S->cfg.rtol;
     d.atol = S->cfg.atol;
     d.max_steps = S->cfg.max_steps;
+    if (o.modes == 1 && o.grad) d.tape_segments = S->cfg.tape_segments;
     fh_plan p = nullptr;
     *rc = fh_plan_create(S->ctx, &d, &p);
     if (*rc == FH_OK && o.grad) *rc = fhi::plan_prepare_gradient(p);
@@ -412,6 +413,7 @@ int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out
     if (cfg->precision != FH_PREC_FP64 && cfg->precision != FH_PREC_FP32_BULK) return fhi::set_error(FH_EINVAL, "unknown precision");
     if (!(cfg->rtol > 0) || !(cfg->atol >= 0) || !(cfg->multimode_step_m > 0))
         return fhi::set_error(FH_EINVAL, "tolerances and the multimode step must be positive");
+    if (cfg->tape_segments < 0) return fhi::set_error(FH_EINVAL, "tape_segments must be >= 0");
     if (hipSetDevice(ctx->device) != hipSuccess) return fhi::set_error(FH_EDEVICE, "hipSetDevice failed");
     fh_server_s* S = new fh_server_s;
     S->ctx = ctx;

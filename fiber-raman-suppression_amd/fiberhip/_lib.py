@@ -93,7 +93,7 @@ class ServerConfig(ctypes.Structure):
         ("atol", ctypes.c_double),
         ("multimode_step_m", ctypes.c_double),
         ("mm_grad_max_steps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("tape_segments", ctypes.c_int32),
     ]
 
 

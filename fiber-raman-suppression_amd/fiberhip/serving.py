@@ -27,17 +27,20 @@ class Server:
     is the number of launches in flight; max_batch the solves per launch.
     Single-mode requests use the adaptive Tsit5 scheme (rtol, atol), multimode
     requests the fixed-step RK4IP with ceil(L / multimode_step_m) steps (at
-    most mm_grad_max_steps for multimode gradients)."""
+    most mm_grad_max_steps for multimode gradients).  Single-mode gradient plans
+    keep the stage tape of their first tape_segments segments of 50 accepted
+    steps (no recomputation there; more device memory per plan)."""
 
     def __init__(self, memory_budget_bytes: int = 64 << 30, lanes: int = 4, max_batch: int = 64,
                  precision: str = "fp32", rtol: float = 1e-6, atol: float = 1e-6, multimode_step_m: float = 1e-4,
-                 max_steps: int = 10000, mm_grad_max_steps: int = 2000, context: Optional[Context] = None):
+                 max_steps: int = 10000, mm_grad_max_steps: int = 2000, context: Optional[Context] = None,
+                 tape_segments: int = 0):
         self.context = context or default_context()
         self.precision = precision
         cfg = _lib.ServerConfig(memory_budget_bytes=int(memory_budget_bytes), lanes=int(lanes),
                                 max_batch=int(max_batch), precision=PRECISIONS[precision], max_steps=int(max_steps),
                                 rtol=float(rtol), atol=float(atol), multimode_step_m=float(multimode_step_m),
-                                mm_grad_max_steps=int(mm_grad_max_steps))
+                                mm_grad_max_steps=int(mm_grad_max_steps), tape_segments=int(tape_segments))
         h = ctypes.c_void_p()
         check(lib().fh_server_create(self.context.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

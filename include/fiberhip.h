@@ -257,7 +257,9 @@ typedef struct {
     double rtol, atol;           /* adaptive tolerances of single-mode requests    */
     double multimode_step_m;     /* fixed step of multimode requests               */
     int32_t mm_grad_max_steps;   /* step capacity of multimode gradients (0 = 2000)*/
-    int32_t reserved;
+    int32_t tape_segments;       /* single-mode gradient plans: accepted-step segments
+                                  * of 50 whose stage tape the forward sweep keeps
+                                  * (fh_problem_desc.tape_segments; 0 = none)      */
 } fh_server_config;
 
 enum { FH_REQ_FORWARD = 0, FH_REQ_GRADIENT = 1 };

@@ -1,6 +1,7 @@
 """GPU tests of the request server (fh_server_*): continuous batching of
 uneven requests, each result bitwise equal to the same request solved alone
-and to the plan API's solve of the same problem; memory-budget admission;
+and to the plan API's solve of the same problem (also with kept stage tape in
+the mixed server's gradient plans); memory-budget admission;
 unsupported request classes rejected at submission."""
 from __future__ import annotations
 
@@ -54,7 +55,10 @@ def test_results_same_alone_and_in_the_mix(gpu_context, precision):
 
     reqs = _requests()
     print(f"case: serving mix {precision}", flush=True)
-    with fiberhip.Server(lanes=3, max_batch=8, precision=precision, multimode_step_m=2.5e-4) as srv:
+    # the mixed server keeps the first tape segment of its single-mode gradients,
+    # the per-request servers and the plan API recompute: results are bitwise equal
+    with fiberhip.Server(lanes=3, max_batch=8, precision=precision, multimode_step_m=2.5e-4,
+                         tape_segments=1) as srv:
         ids = [None] * len(reqs)
 
         def feed(start):
