@@ -64,9 +64,10 @@ template <typename R> struct MmArgs {
     double* grad;   // [batch][N] sum_m 2 Re(conj(lam0_m) i u0_m)
     double* cost;   // [batch]
     C2<R>* ckpt;    // [batch][nseg][M][N]
-    C2<R>* tape_u;  // [batch][4 every][M][N]
-    C2<R>* tape_h;  // [batch][4 every][QC][N]
+    C2<R>* tape_u;  // [batch][regions][4 every][M][N]
+    C2<R>* tape_h;  // [batch][regions][4 every][QC][N]
     int every, nseg;
+    int ntaped;  // the last ntaped segments of each solve keep the forward sweep's tape (k_grad_multi)
 };
 
 template <typename R, int N, int T, int M> struct MmTraits {
@@ -709,10 +710,11 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     const int Q = a.q, QC = (a.q + 1) / 2;
     const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, Q, QC};
     const Buf ckpt(a.ckpt + sol * (size_t)a.nseg * M * N, (uint32_t)((size_t)a.nseg * M * N * sizeof(V)));
-    const Buf tape_u(a.tape_u + sol * (size_t)4 * a.every * M * N,
-                     (uint32_t)((size_t)4 * a.every * M * N * sizeof(V)));
-    const Buf tape_h(a.tape_h + sol * (size_t)4 * a.every * QC * N,
-                     (uint32_t)((size_t)4 * a.every * QC * N * sizeof(V)));
+    // stage tape: one region of 4*every slots per kept segment plus the shared
+    // region of the recomputed segments (when the plan recomputes any)
+    const size_t tslots = (size_t)(a.ntaped + (a.ntaped < a.nseg ? 1 : 0)) * 4 * a.every;
+    const Buf tape_u(a.tape_u + sol * tslots * M * N, (uint32_t)(tslots * M * N * sizeof(V)));
+    const Buf tape_h(a.tape_h + sol * tslots * QC * N, (uint32_t)(tslots * QC * N * sizeof(V)));
     const Buf none(a.u0, 0);
     const uint32_t vo = CV::voff();
     // per-solve length and step count (the plan's steps bound the checkpoint capacity)
@@ -720,6 +722,9 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     const int Sb = a.nsteps ? a.nsteps[sol] : a.steps;
     const double h = a.lengths ? Lb / Sb : a.h;
     const int nseg = (Sb + a.every - 1) / a.every;
+    // the last ntaped segments of this solve keep the forward sweep's tape
+    const int first_kept = nseg - min(a.ntaped, nseg);
+    auto seg_base = [&](int sg) { return (uint32_t)((sg >= first_kept ? sg - first_kept : a.ntaped) * 4 * a.every); };
     V v[P];
     for (int m = 0; m < M; ++m) {
         const Fld y = c.fld(m), st = c.fld(M + m);
@@ -730,8 +735,9 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
             st.st(x, vo, CV::soff(k));
         }
     }
-    // forward with checkpoints of the state
-    for (int step = 0; step < Sb; ++step) {
+    // forward with checkpoints of the state; kept segments tape their stages
+    const int kept_from = min(first_kept * a.every, Sb);
+    for (int step = 0; step < kept_from; ++step) {
         if (step % a.every == 0) {
             drain_memory();
             for (int m = 0; m < M; ++m) {
@@ -745,6 +751,14 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
 #pragma unroll 1
         for (int s = 0; s < 4; ++s)
             mm_stage<R, N, T, M, false>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, none, none, 0, v, lds);
+    }
+    for (int step = kept_from; step < Sb; ++step) {
+        const double z = step * h;
+        const int sg = step / a.every;
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s)
+            mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
+                                       seg_base(sg) + (uint32_t)(4 * (step - sg * a.every) + s), v, lds);
     }
     drain_memory();
     // output and terminal seed (band energy over all modes)
@@ -800,23 +814,26 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * a.every;
         const int s1 = s0 + a.every < Sb ? s0 + a.every : Sb;
+        const uint32_t base = seg_base(sg);
         drain_memory();
-        for (int m = 0; m < M; ++m) {
-            const Fld ck{ckpt, ((uint32_t)sg * (uint32_t)M + (uint32_t)m) * FB};
-            const Fld y = c.fld(m), st = c.fld(M + m);
+        if (sg < first_kept) {  // recompute the segment's tape from its checkpoint
+            for (int m = 0; m < M; ++m) {
+                const Fld ck{ckpt, ((uint32_t)sg * (uint32_t)M + (uint32_t)m) * FB};
+                const Fld y = c.fld(m), st = c.fld(M + m);
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const V x = ck.template ld<V>(vo, CV::soff(k));
-                y.st(x, vo, CV::soff(k));
-                st.st(x, vo, CV::soff(k));
+                for (int k = 0; k < P; ++k) {
+                    const V x = ck.template ld<V>(vo, CV::soff(k));
+                    y.st(x, vo, CV::soff(k));
+                    st.st(x, vo, CV::soff(k));
+                }
             }
-        }
-        for (int step = s0; step < s1; ++step) {
-            const double z = step * h;
+            for (int step = s0; step < s1; ++step) {
+                const double z = step * h;
 #pragma unroll 1
-            for (int s = 0; s < 4; ++s)
-                mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
-                                           (uint32_t)(4 * (step - s0) + s), v, lds);
+                for (int s = 0; s < 4; ++s)
+                    mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
+                                               base + (uint32_t)(4 * (step - s0) + s), v, lds);
+            }
         }
         for (int step = s1 - 1; step >= s0; --step) {
             const double z = step * h;
@@ -825,7 +842,7 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
                 const double zs = s == 0 ? z : (s == 3 ? z + h : z + h / 2);
                 const R cl = (s == 0 || s == 3) ? c6 : c3;
                 const R cg = s == 3 ? (R)0 : (s == 2 ? c1 : c2);
-                mm_vjp<R, N, T, M>(c, zs, cl, cg, tape_u, tape_h, (uint32_t)(4 * (step - s0) + s), v, lds);
+                mm_vjp<R, N, T, M>(c, zs, cl, cg, tape_u, tape_h, base + (uint32_t)(4 * (step - s0) + s), v, lds);
                 drain_memory();
                 for (int m = 0; m < M; ++m) {
                     const Fld lam = c.fld(TR::LAM + m), lacc = c.fld(TR::LACC + m), gx = c.fld(TR::GX + m);

@@ -256,8 +256,8 @@ int check_desc(const fh_problem_desc* d)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->tape_segments < 0) return fail(FH_EINVAL, "tape_segments must be >= 0");
-    if (d->tape_segments > 0 && d->modes != 1)
-        return fail(FH_EINVAL, "tape_segments applies to the single-mode gradients");
+    if (d->tape_segments > 0 && d->modes != 1 && d->scheme != FH_SCHEME_RK4IP_FIXED)
+        return fail(FH_EINVAL, "tape_segments applies to the single-mode gradients and the fixed-step multimode one");
     if (d->modes == 1) {
         const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
         if (d->nt < 64 || d->nt > nmax)
@@ -450,10 +450,11 @@ template <typename R> int ensure_adjoint_buffers(fh_plan p)
     // always, since the segment count is known only at the end of a solve)
     const size_t regions = (size_t)p->ntaped + ((p->adaptive || p->ntaped < p->nseg) ? 1 : 0);
     const size_t solve_tape = V * nt * M * stages * (size_t)p->every * regions;
-    if (solve_tape > 0xffffffffull)
+    const size_t solve_tape_h = hbytes * stages * (size_t)p->every * regions;
+    if (solve_tape > 0xffffffffull || solve_tape_h > 0xffffffffull)
         return fail(FH_EINVAL, "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every");
     if ((rc = dev_alloc(p, p->tape_u, solve_tape * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_h, hbytes * stages * (size_t)p->every * regions * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, solve_tape_h * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_lam0, V * nt * M * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;
@@ -505,6 +506,7 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
         m.tape_h = (fh::C2<R>*)p->tape_h.p;
         m.every = p->every;
         m.nseg = p->nseg;
+        m.ntaped = p->ntaped;
         m.lengths = d_lengths;
         m.nsteps = d_nsteps;  // each at most the plan's steps (checkpoint capacity)
         FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));

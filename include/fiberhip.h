@@ -93,15 +93,17 @@ typedef struct {
     double rtol;              /* FH_SCHEME_TSIT5: relative tolerance (reltol) */
     double atol;              /* FH_SCHEME_TSIT5: absolute tolerance (abstol) */
     int32_t max_steps;        /* FH_SCHEME_TSIT5: accepted-step limit (0 = 10000) */
-    int32_t tape_segments;    /* single-mode adjoint: K checkpoint segments keep the
-                               * stage tape of the forward sweep, so the backward sweep
-                               * recomputes only the others: the last K of a fixed-step
-                               * solve, the first K accepted-step segments of an adaptive
-                               * one (0 = none; clamped to the segment count of steps or
-                               * max_steps).  Device memory grows by K * checkpoint_every
-                               * * stages (4 fixed, 6 adaptive) * nt * (complex + real)
-                               * per solve of max_batch.  Results are bit-identical for
-                               * every K. */
+    int32_t tape_segments;    /* adjoint: K checkpoint segments keep the stage tape of
+                               * the forward sweep, so the backward sweep recomputes only
+                               * the others: the last K of a fixed-step solve (single or
+                               * multimode), the first K accepted-step segments of an
+                               * adaptive single-mode one (0 = none; clamped to the
+                               * segment count of steps or max_steps; adaptive multimode
+                               * plans reject it).  Device memory grows by K *
+                               * checkpoint_every * stages (4 fixed, 6 adaptive) * the
+                               * stage's tape (single mode: nt complex + nt real;
+                               * multimode: (M + ceil(Q/2)) * nt complex) per solve of
+                               * max_batch.  Results are bit-identical for every K. */
 } fh_problem_desc;
 
 /* Counters of one call, summed over its batch (per-solve values: fh_solve_stats). */
@@ -167,8 +169,8 @@ int fh_forward_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL,
  * gradient, accumulated in FP64), cost (J per launch; NaN with a seed).
  * Forward states are checkpointed every checkpoint_every steps in device
  * memory and each segment is recomputed once during the backward sweep,
- * except the tape_segments segments of a single-mode plan whose stage tape
- * the forward sweep keeps. */
+ * except the tape_segments segments whose stage tape the forward sweep
+ * keeps. */
 int fh_gradient(fh_plan plan, int32_t batch, const double *u0, const double *seed, double *uL, double *lam0,
                 double *grad, double *cost, fh_stats *stats);
 
