@@ -195,3 +195,27 @@ def test_error_codes(gpu_context):
     with pytest.raises(fiberhip.FiberHIPError) as err:
         fiberhip.Plan.from_problem(problem, steps=0)
     assert err.value.code == _lib.FH_EINVAL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_nonfinite_solution_reports_enonfinite(gpu_context, precision):
+    """A launch scaled until the Kerr term overflows: the fixed-step forward,
+    the adaptive forward (which stops at the first non-finite error estimate)
+    and the gradient all return FH_ENONFINITE, the reference's failed-retcode
+    error (scalar_crosscheck.jl:66, propagation.jl:48-53)."""
+    import fiberhip
+    from fiberhip import _lib
+
+    problem = smf_problem(nt=256, window_ps=3.0, length_m=0.05, policy="exact")
+    u = problem.uω0 * (1e160 if precision == "fp64" else 1e18)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    print(f"case: non-finite forward/gradient {precision}", flush=True)
+    for kw in ({"steps": 8}, {"scheme": "tsit5"}):
+        plan = fiberhip.Plan.from_problem(problem, precision=precision, **kw)
+        with pytest.raises(fiberhip.FiberHIPError) as err:
+            plan.forward(u)
+        assert err.value.code == _lib.FH_ENONFINITE, kw
+    plan = fiberhip.Plan.from_problem(problem, steps=8, precision=precision, band_mask=mask)
+    with pytest.raises(fiberhip.FiberHIPError) as err:
+        plan.gradient(u)
+    assert err.value.code == _lib.FH_ENONFINITE
