@@ -55,9 +55,15 @@ template <typename R, int N, int T, int M> struct MtgLayout {
 // 2 every, ... is stored to ckpt (M fields per checkpoint); without CKPT the
 // dense-output samples of a.zsave are written.  Returns the workgroup-uniform
 // status: 0 ok, 1 non-finite, 2 step limit.
+// With CKPT every stage evaluation also stores its u and h pairs to the stage
+// tape: slot 6 (n mod every) + i of accepted step n, in the region of segment
+// n / every when that segment is one of the first ntaped (kept), else in the
+// shared region the backward sweep recomputes into.  A rejected attempt is
+// overwritten by the next one; the seventh stage is stage 0 of step n + 1.
 template <typename R, int N, int T, int M, bool CKPT>
-__device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const Buf& ckpt, int* ys_out,
-                          long long* nacc_out, long long* nrej_out, long long* nfe_out, C2<R>* v, void* lds)
+__device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const Buf& ckpt, const Buf& tape_u,
+                          const Buf& tape_h, int* ys_out, long long* nacc_out, long long* nrej_out,
+                          long long* nfe_out, C2<R>* v, void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
@@ -76,6 +82,11 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
     const Buf none(ma.u0, 0);
     int sy = LY::Y, syn = LY::YN, sk0 = LY::K0, sk6 = LY::K0 + 6 * M;
     auto kslot = [&](int j) { return j == 0 ? sk0 : (j == 6 ? sk6 : LY::K0 + j * M); };
+    auto tape_slot = [&](long long n, int i) {
+        const int sg = (int)(n / ma.every);
+        const int region = sg < ma.ntaped ? sg : ma.ntaped;
+        return (uint32_t)(region * 6 * ma.every + 6 * (int)(n % ma.every) + i);
+    };
     long long nfe = 0, nacc = 0, nrej = 0;
 
     // dense output: lab-frame sample of every mode at zj (see ts_save)
@@ -114,7 +125,8 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
     };
 
     // k0 = f(0, y0) and the starting-step norms (Hairer-Norsett-Wanner II.4)
-    mm_rhs<R, N, T, M>(c, 0.0, sy, sk0, LY::U, LY::HQ, none, none, 0, v, lds);
+    mm_rhs<R, N, T, M, CKPT>(c, 0.0, sy, sk0, LY::U, LY::HQ, CKPT ? tape_u : none, CKPT ? tape_h : none,
+                             CKPT ? tape_slot(0, 0) : 0u, v, lds);
     nfe++;
     drain_memory();
     double s0 = 0, s1 = 0;
@@ -215,7 +227,9 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
                     yn.st(V{u.x + hr * s.x, u.y + hr * s.y}, vo, CV::soff(k));
                 }
             }
-            mm_rhs<R, N, T, M>(c, z + TB::c[i] * h, syn, kslot(i), LY::U, LY::HQ, none, none, 0, v, lds);
+            const uint32_t ts = CKPT ? (i < 6 ? tape_slot(nacc, i) : tape_slot(nacc + 1, 0)) : 0u;
+            mm_rhs<R, N, T, M, CKPT>(c, z + TB::c[i] * h, syn, kslot(i), LY::U, LY::HQ, CKPT ? tape_u : none,
+                                     CKPT ? tape_h : none, ts, v, lds);
         }
         nfe += 6;
         drain_memory();
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
     int ys;
     long long nacc, nrej, nfe;
     const Buf none(ma.u0, 0);
-    const int status = mt_forward<R, N, T, M, false>(a, c, none, &ys, &nacc, &nrej, &nfe, v, lds);
+    const int status = mt_forward<R, N, T, M, false>(a, c, none, none, none, &ys, &nacc, &nrej, &nfe, v, lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
@@ -366,10 +380,11 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     const int Q = ma.q, QC = (ma.q + 1) / 2;
     const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, Q, QC};
     const Buf ckpt(ma.ckpt + sol * (size_t)ma.nseg * M * N, (uint32_t)((size_t)ma.nseg * M * N * sizeof(V)));
-    const Buf tape_u(ma.tape_u + sol * (size_t)6 * ma.every * M * N,
-                     (uint32_t)((size_t)6 * ma.every * M * N * sizeof(V)));
-    const Buf tape_h(ma.tape_h + sol * (size_t)6 * ma.every * QC * N,
-                     (uint32_t)((size_t)6 * ma.every * QC * N * sizeof(V)));
+    // stage tape: one region of 6*every slots per kept segment (the first
+    // ntaped) plus the shared region of the recomputed segments
+    const size_t tslots = (size_t)(ma.ntaped + 1) * 6 * ma.every;
+    const Buf tape_u(ma.tape_u + sol * tslots * M * N, (uint32_t)(tslots * M * N * sizeof(V)));
+    const Buf tape_h(ma.tape_h + sol * tslots * QC * N, (uint32_t)(tslots * QC * N * sizeof(V)));
     const uint32_t vo = CV::voff();
     const double L = ma.lengths ? ma.lengths[sol] : ma.length;
     V v[P];
@@ -381,7 +396,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     }
     int ys;
     long long nacc, nrej, nfe;
-    int bad = mt_forward<R, N, T, M, true>(a, c, ckpt, &ys, &nacc, &nrej, &nfe, v, lds);
+    int bad = mt_forward<R, N, T, M, true>(a, c, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, v, lds);
     if (threadIdx.x == 0) {
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
@@ -446,16 +461,18 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int s0 = sg * ma.every;
         const int s1 = (long long)s0 + ma.every < nacc ? s0 + ma.every : (int)nacc;
+        const bool kept = sg < ma.ntaped;  // the forward sweep left this segment's tape
+        const uint32_t rbase = (uint32_t)((kept ? sg : ma.ntaped) * 6 * ma.every);
         drain_memory();
 #pragma unroll 1
-        for (int m = 0; m < M; ++m) {
+        for (int m = 0; m < (kept ? 0 : M); ++m) {
             const Fld ck{ckpt, ((uint32_t)sg * (uint32_t)M + (uint32_t)m) * FB};
             const Fld y = c.fld(LB::Y + m);
 #pragma unroll
             for (int k = 0; k < P; ++k) y.st(ck.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
         }
         // recompute the segment with stage tapes; stage inputs in gx
-        for (int s = s0; s < s1; ++s) {
+        for (int s = s0; s < (kept ? s0 : s1); ++s) {
             const double h = a.hs[sol * a.max_steps + s], z = a.zs[sol * a.max_steps + s];
             const R hr = (R)h;
 #pragma unroll 1
@@ -482,7 +499,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
                     }
                 }
                 mm_rhs<R, N, T, M, true>(c, z + TB::c[i] * h, i == 0 ? LB::Y : TR::GX, LB::K0 + i * M, LB::U,
-                                         LB::HQ, tape_u, tape_h, (uint32_t)(6 * (s - s0) + i), v, lds);
+                                         LB::HQ, tape_u, tape_h, rbase + (uint32_t)(6 * (s - s0) + i), v, lds);
             }
             if (s + 1 < s1) {
                 drain_memory();
@@ -538,7 +555,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
                     }
                 }
                 mm_vjp<R, N, T, M>(c, z + TB::c[i] * h, hr * (R)TB::a[6][i], i < 5 ? (R)1 : (R)0, tape_u, tape_h,
-                                   (uint32_t)(6 * (s - s0) + i), v, lds);
+                                   rbase + (uint32_t)(6 * (s - s0) + i), v, lds);
                 drain_memory();
 #pragma unroll 1
                 for (int m = 0; m < M; ++m) {

@@ -256,8 +256,6 @@ int check_desc(const fh_problem_desc* d)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->tape_segments < 0) return fail(FH_EINVAL, "tape_segments must be >= 0");
-    if (d->tape_segments > 0 && d->modes != 1 && d->scheme != FH_SCHEME_RK4IP_FIXED)
-        return fail(FH_EINVAL, "tape_segments applies to the single-mode gradients and the fixed-step multimode one");
     if (d->modes == 1) {
         const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
         if (d->nt < 64 || d->nt > nmax)

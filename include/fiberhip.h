@@ -95,11 +95,10 @@ typedef struct {
     int32_t max_steps;        /* FH_SCHEME_TSIT5: accepted-step limit (0 = 10000) */
     int32_t tape_segments;    /* adjoint: K checkpoint segments keep the stage tape of
                                * the forward sweep, so the backward sweep recomputes only
-                               * the others: the last K of a fixed-step solve (single or
-                               * multimode), the first K accepted-step segments of an
-                               * adaptive single-mode one (0 = none; clamped to the
-                               * segment count of steps or max_steps; adaptive multimode
-                               * plans reject it).  Device memory grows by K *
+                               * the others: the last K of a fixed-step solve, the first
+                               * K accepted-step segments of an adaptive one (0 = none;
+                               * clamped to the segment count of steps or max_steps).
+                               * Device memory grows by K *
                                * checkpoint_every * stages (4 fixed, 6 adaptive) * the
                                * stage's tape (single mode: nt complex + nt real;
                                * multimode: (M + ceil(Q/2)) * nt complex) per solve of

@@ -1,8 +1,8 @@
 """Kept tape segments of the gradients (tape_segments).
 
 The forward sweep stores the stage tape of K checkpoint segments (the last K
-of a fixed-step solve, single or multimode; the first K of an adaptive
-single-mode one) and the backward sweep recomputes only the others.  The recomputation repeats
+of a fixed-step solve, the first K of an adaptive one; single and multimode)
+and the backward sweep recomputes only the others.  The recomputation repeats
 the forward sweep's arithmetic from the checkpoint, so every output (uL,
 cost, lam0, grad) must be bit-identical for every K, in both precisions,
 including a partial last segment and K at or beyond the segment count.  The
@@ -124,9 +124,35 @@ def test_tape_segments_validation(gpu_context):
     with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
         fiberhip.Plan.from_problem(problem, steps=60, tape_segments=-1)
     gp = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.002, energy_j=1e-9, nx=96, window_um=60.0)
-    fiberhip.Plan.from_problem(gp, steps=10, tape_segments=1)  # fixed-step multimode: accepted
-    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
-        fiberhip.Plan.from_problem(gp, scheme="tsit5", tape_segments=1)
+    fiberhip.Plan.from_problem(gp, steps=10, tape_segments=1)  # every gradient accepts it
+    fiberhip.Plan.from_problem(gp, scheme="tsit5", tape_segments=1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adaptive_multimode_kept_tape_is_bit_identical(gpu_context, precision):
+    """Adaptive multimode gradient (GRIN M=3): the first K segments of 7
+    accepted steps keep the forward sweep's stage tape."""
+    import fiberhip
+    from fiberhip import grin
+
+    p = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.02, energy_j=3e-9, nx=96, window_um=60.0)
+    mask = fiberhip.frequency_band_mask(p, (-16.0, -4.0))
+    launches = np.stack([grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.02, energy_j=3e-9, nx=96,
+                                           window_um=60.0, seed=s).uω0 for s in (1, 2)])
+    res = {}
+    for k in (0, 1, 400):
+        print(f"case: adaptive multimode tape_segments={k} M=3 nt=512 every=7 {precision}", flush=True)
+        plan = fiberhip.Plan.from_problem(p, scheme="tsit5", rtol=1e-7, atol=1e-6, precision=precision,
+                                          band_mask=mask, checkpoint_every=7, max_steps=2000, max_batch=2,
+                                          tape_segments=k)
+        res[k] = plan.gradient(launches)
+        steps = plan.solve_stats(2)[:, 0]
+        del plan
+    print("accepted steps", steps, flush=True)
+    assert steps.min() > 7  # at least two segments
+    for k in (1, 400):
+        for key in ("uL", "cost", "lam0", "grad"):
+            assert np.array_equal(res[k][key], res[0][key]), (precision, k, key)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
