@@ -104,10 +104,13 @@ def _band_objective(name, mask, log_cost):
 
 
 class AdjointModel:
-    """GPU spectral-shaper model: launch alpha*A*cis(phi)*uw0 (here alpha = A = 1)."""
+    """GPU spectral-shaper model: launch alpha*A*cis(phi)*uw0 (here alpha = A = 1).
+
+    tape_segments: checkpoint segments whose stage tape the forward sweep keeps
+    (fewer recomputations, more device memory; gradients are bit-identical)."""
 
     def __init__(self, problem: FiberFieldProblem, steps: int = 2000, precision: str = "fp64",
-                 checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None):
+                 checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None, tape_segments: int = 0):
         g = np.asarray(problem.fiber["γ"])
         if not _gamma_symmetric(g):
             raise ValueError("fiber adjoint requires a fully permutation-symmetric gamma tensor; "
@@ -117,7 +120,7 @@ class AdjointModel:
         self.precision = precision
         mask = objective.band_mask if objective is not None else problem.band_mask
         self.plan = Plan.from_problem(problem, steps=steps, precision=precision, band_mask=mask,
-                                      checkpoint_every=checkpoint_every,
+                                      checkpoint_every=checkpoint_every, tape_segments=tape_segments,
                                       log_cost=bool(objective.log_cost) if objective is not None else False)
 
     def shaped_input(self, phase) -> np.ndarray:
@@ -142,8 +145,9 @@ def _gamma_symmetric(g) -> bool:
 
 
 def fiber_model(problem: FiberFieldProblem, steps: int = 2000, precision: str = "fp64",
-                checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None) -> AdjointModel:
-    return AdjointModel(problem, steps, precision, checkpoint_every, objective)
+                checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None,
+                tape_segments: int = 0) -> AdjointModel:
+    return AdjointModel(problem, steps, precision, checkpoint_every, objective, tape_segments)
 
 
 @dataclasses.dataclass
