@@ -96,6 +96,11 @@ class Plan:
 
     dispersion (nt, m) rad/m, gamma (m, m, m, m), raman (nt,) complex hR(ω),
     shock (nt,) = fftshift(ωs/ω0), all in FFT order as the reference builds them.
+    Gradients checkpoint every `checkpoint_every` steps; `tape_segments`
+    checkpoint segments keep the forward sweep's stage tape instead of being
+    recomputed (the last ones of a fixed-step solve, the first ones of an
+    adaptive one): fewer transforms per gradient for more device memory, with
+    bit-identical results.
     """
 
     def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: Optional[int],
