@@ -170,3 +170,26 @@ def test_grin6_launch():
     ut = np.fft.fft(p.uω0, axis=0)
     assert math.isclose(np.max(np.sum(np.abs(ut) ** 2, axis=1)), 0.881374 * 10e-9 / 100e-15, rel_tol=1e-12)
     assert p.uω0.shape == (4096, 6)
+
+
+def test_bench_kept_segment_budget():
+    """bench.py spends one tape budget on both precisions: kept segments per
+    gradient = budget / (batch * 4 * every * nt * 3 * element) - 1 (one region
+    serves the recomputed segments), clamped to [0, segments]."""
+    import importlib.util
+    import pathlib
+
+    path = pathlib.Path(__file__).resolve().parents[1] / "bench.py"
+    spec = importlib.util.spec_from_file_location("bench_for_test", str(path))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    gib = 1 << 30
+    # SMF-28 2 m gradient: 16384 samples, 2000 steps, checkpoint every 50 (40 segments), 256 gradients
+    assert bench.kept_segments(16384, 2000, 50, "fp32", 256, 200 * gib) == 20
+    assert bench.kept_segments(16384, 2000, 50, "fp64", 256, 200 * gib) == 9
+    assert bench.kept_segments(16384, 2000, 50, "fp32", 256, 0) == 0
+    assert bench.kept_segments(16384, 2000, 50, "fp32", 1, 200 * gib) == 40
+    # the regions fit the budget: (kept + shared) * region bytes <= budget
+    for prec, esz in (("fp32", 4), ("fp64", 8)):
+        k = bench.kept_segments(16384, 2000, 50, prec, 256, 200 * gib)
+        assert (k + 1) * 4 * 50 * 16384 * 3 * esz * 256 <= 200 * gib
