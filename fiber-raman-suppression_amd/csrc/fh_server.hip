@@ -380,7 +380,8 @@ int validate(const fh_server_s* S, const fh_request* r, double* dmax)
     if (!(std::isfinite(r->length_m) && r->length_m > 0)) return fhi::set_error(FH_EINVAL, "fiber length must be positive and finite");
     if (r->kind == FH_REQ_GRADIENT) {
         if (!r->band_mask) return fhi::set_error(FH_EINVAL, "gradient requests need a band mask");
-        if (r->modes > 1 && std::ceil(r->length_m / S->cfg.multimode_step_m) > mm_grad_cap(S))
+        // the step count submitted below (at least 16) must fit the plan's checkpoint capacity
+        if (r->modes > 1 && std::max(16.0, std::ceil(r->length_m / S->cfg.multimode_step_m)) > mm_grad_cap(S))
             return fhi::set_error(FH_EUNSUPPORTED, "multimode gradient needs more steps than the server's capacity");
     }
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;

@@ -127,6 +127,10 @@ def test_memory_budget_admission_and_rejections(gpu_context):
         five = grin.grin_problem(5, nt=512, time_window_ps=4.0, length_m=0.01, energy_j=1e-9, nx=96, window_um=60.0)
         with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):  # no kernel for M = 5
             tight.submit(five)
+    # a short multimode gradient still runs at least 16 steps: beyond a 10-step capacity
+    with fiberhip.Server(lanes=1, max_batch=2, precision="fp32", multimode_step_m=1e-4, mm_grad_max_steps=10) as cap:
+        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
+            cap.submit(mm, kind="gradient", band_mask=np.ones(512, dtype=bool), length_m=5e-4)
     with fiberhip.Server(lanes=1, max_batch=4, precision="fp32", memory_budget_bytes=1 << 20) as starved:
         with pytest.raises(fiberhip.FiberHIPError, match="FH_ENOMEM"):
             starved.wait(_submit(starved, reqs[1]))
