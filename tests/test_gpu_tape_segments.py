@@ -176,3 +176,28 @@ def test_multimode_kept_tape_is_bit_identical(gpu_context, precision):
     for k in (2, 6, 10):
         for key in ("uL", "cost", "lam0", "grad"):
             assert np.array_equal(res[k][key], res[0][key]), (precision, k, key)
+
+
+@pytest.mark.parametrize("scheme,nt,window,length,power", [("rk4ip", 32768, 94.0, 0.05, 0.1),
+                                                          ("tsit5", 16384, 47.061, 0.5, 0.5)])
+def test_kept_tape_large_grids_bit_identical(gpu_context, scheme, nt, window, length, power):
+    """The 1024-thread FP32 kernels: fixed step at 32768 samples (32 per
+    thread, planar exchange; 40 steps in 7 segments) and adaptive at 16384
+    samples (about 320 accepted steps with rejections, segments of 6), with
+    and without kept segments."""
+    import fiberhip
+
+    problem = smf_problem(nt=nt, window_ps=window, length_m=length, power_w=power)
+    mask = fiberhip.frequency_band_mask(problem, (-16.0, -10.0))
+    launches = _launches(problem, 2, 13)
+    res = {}
+    for k in (0, 2):
+        print(f"case: {scheme} tape_segments={k} nt={nt} fp32", flush=True)
+        kw = {"steps": 40} if scheme == "rk4ip" else {"scheme": "tsit5", "rtol": 1e-8, "atol": 1e-6,
+                                                      "max_steps": 2000}
+        plan = fiberhip.Plan.from_problem(problem, precision="fp32", band_mask=mask, checkpoint_every=6, max_batch=2,
+                                          tape_segments=k, **kw)
+        res[k] = plan.gradient(launches)
+        del plan
+    for key in ("uL", "cost", "lam0", "grad"):
+        assert np.array_equal(res[2][key], res[0][key]), (scheme, key)
