@@ -107,10 +107,14 @@ class AdjointModel:
     """GPU spectral-shaper model: launch alpha*A*cis(phi)*uw0 (here alpha = A = 1).
 
     tape_segments: checkpoint segments whose stage tape the forward sweep keeps
-    (fewer recomputations, more device memory; gradients are bit-identical)."""
+    (fewer recomputations, more device memory; gradients are bit-identical).
+    None keeps as many as fit in TAPE_BUDGET_BYTES (one gradient per call)."""
+
+    TAPE_BUDGET_BYTES = 4 << 30
 
     def __init__(self, problem: FiberFieldProblem, steps: int = 2000, precision: str = "fp64",
-                 checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None, tape_segments: int = 0):
+                 checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None,
+                 tape_segments: Optional[int] = None):
         g = np.asarray(problem.fiber["γ"])
         if not _gamma_symmetric(g):
             raise ValueError("fiber adjoint requires a fully permutation-symmetric gamma tensor; "
@@ -119,9 +123,28 @@ class AdjointModel:
         self.steps = steps
         self.precision = precision
         mask = objective.band_mask if objective is not None else problem.band_mask
+        if tape_segments is None:
+            tape_segments = self.default_tape_segments(problem, steps, precision, checkpoint_every)
+        self.tape_segments = tape_segments
         self.plan = Plan.from_problem(problem, steps=steps, precision=precision, band_mask=mask,
                                       checkpoint_every=checkpoint_every, tape_segments=tape_segments,
                                       log_cost=bool(objective.log_cost) if objective is not None else False)
+
+    @classmethod
+    def default_tape_segments(cls, problem: FiberFieldProblem, steps: int, precision: str,
+                              checkpoint_every: int) -> int:
+        """Kept segments whose regions (plus the shared recomputation region)
+        fit in TAPE_BUDGET_BYTES: a stage of the tape holds u (complex) and
+        h (real, single mode) or u of every mode and ceil(Q/2) complex h pairs
+        (multimode), 4 stages per step."""
+        esz = 8 if precision == "fp64" else 4
+        nt, m = problem.nt, problem.modes
+        every = max(1, min(int(checkpoint_every), int(steps)))
+        q = m * (m + 1) // 2
+        per_stage = nt * 3 * esz if m == 1 else nt * (m + (q + 1) // 2) * 2 * esz
+        per_segment = 4 * every * per_stage
+        nseg = -(-int(steps) // every)
+        return int(max(0, min(nseg, cls.TAPE_BUDGET_BYTES // per_segment - 1)))
 
     def shaped_input(self, phase) -> np.ndarray:
         """_shaped_input, physics_models.jl:791-797 (alpha = A = 1)."""
@@ -146,7 +169,7 @@ def _gamma_symmetric(g) -> bool:
 
 def fiber_model(problem: FiberFieldProblem, steps: int = 2000, precision: str = "fp64",
                 checkpoint_every: int = 50, objective: Optional[ObjectiveMap] = None,
-                tape_segments: int = 0) -> AdjointModel:
+                tape_segments: Optional[int] = None) -> AdjointModel:
     return AdjointModel(problem, steps, precision, checkpoint_every, objective, tape_segments)
 
 

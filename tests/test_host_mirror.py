@@ -193,3 +193,21 @@ def test_bench_kept_segment_budget():
     for prec, esz in (("fp32", 4), ("fp64", 8)):
         k = bench.kept_segments(16384, 2000, 50, prec, 256, 200 * gib)
         assert (k + 1) * 4 * 50 * 16384 * 3 * esz * 256 <= 200 * gib
+
+
+def test_adjoint_model_default_tape_budget():
+    """AdjointModel keeps as many tape segments as fit in its 4 GiB budget:
+    every segment of the SMF-28 2 m gradient (N = 16384, 2000 steps, 40
+    segments) in both precisions, and a bounded count for a long solve."""
+    from conftest import smf_problem
+    from fiberhip.adjoint import AdjointModel
+
+    p = smf_problem(nt=16384, window_ps=47.061, length_m=2.0, power_w=0.2)
+    assert AdjointModel.default_tape_segments(p, 2000, "fp64", 50) == 40
+    assert AdjointModel.default_tape_segments(p, 2000, "fp32", 50) == 40
+    # 20000 steps: 400 segments; 4 GiB holds 54 FP64 regions (78.6 MB) and 109 FP32 ones
+    assert AdjointModel.default_tape_segments(p, 20000, "fp64", 50) == 53
+    assert AdjointModel.default_tape_segments(p, 20000, "fp32", 50) == 108
+    for prec, esz in (("fp64", 8), ("fp32", 4)):
+        k = AdjointModel.default_tape_segments(p, 20000, prec, 50)
+        assert (k + 1) * 4 * 50 * 16384 * 3 * esz <= AdjointModel.TAPE_BUDGET_BYTES
