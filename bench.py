@@ -29,7 +29,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense vector/matrix peaks
 # (scaled x2.0 / x1.0 by a plain copy kernel of known size); the summaries and
 # commands are in profiles/.
 MEASURED_HBM_BYTES = {
-    "k_grad_single<float,16384,1024>": (1.5433e10, "profiles/k_grad_single_smf28_16384.md"),
+    "k_grad_single<float,16384,1024>": (1.1217e10, "profiles/k_grad_single_smf28_16384.md"),
     "k_rk4ip_single<float,8192,512>": (1.3921e5, "profiles/k_rk4ip_single_smf28_8192.md"),
     "k_rk4ip_multi<float,4096,512,6>": (4.6848e10, "profiles/k_rk4ip_multi_grin6_4096.md"),
 }
