@@ -134,6 +134,9 @@ struct fh_server_s {
 
 namespace {
 
+// Largest fixed-step count of a multimode request.
+constexpr int kMaxMultimodeSteps = 1 << 22;
+
 int mm_grad_cap(const fh_server_s* S) { return S->cfg.mm_grad_max_steps > 0 ? S->cfg.mm_grad_max_steps : 2000; }
 
 void finish(fh_server_s* S, int64_t id, int code)
@@ -378,6 +381,11 @@ int validate(const fh_server_s* S, const fh_request* r, double* dmax)
     if (!r->dispersion || !r->gamma || !r->raman || !r->shock || !r->u0 || !r->uL)
         return fhi::set_error(FH_EINVAL, "operator, launch or output pointer is NULL");
     if (!(std::isfinite(r->length_m) && r->length_m > 0)) return fhi::set_error(FH_EINVAL, "fiber length must be positive and finite");
+    // multimode requests run ceil(L / multimode_step_m) fixed steps: bound the
+    // count before it is converted to an int
+    if (r->modes > 1 && !(std::ceil(r->length_m / S->cfg.multimode_step_m) <= (double)kMaxMultimodeSteps))
+        return fhi::set_error(FH_EUNSUPPORTED, "multimode request needs more than " +
+                                                   std::to_string(kMaxMultimodeSteps) + " fixed steps");
     if (r->kind == FH_REQ_GRADIENT) {
         if (!r->band_mask) return fhi::set_error(FH_EINVAL, "gradient requests need a band mask");
         // the step count submitted below (at least 16) must fit the plan's checkpoint capacity

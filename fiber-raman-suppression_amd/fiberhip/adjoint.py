@@ -139,7 +139,9 @@ class AdjointModel:
         (multimode), 4 stages per step."""
         esz = 8 if precision == "fp64" else 4
         nt, m = problem.nt, problem.modes
-        every = max(1, min(int(checkpoint_every), int(steps)))
+        # the plan's rule (fh_plan_create): checkpoint_every = 0 means 50,
+        # and the spacing never exceeds the step count
+        every = min(int(checkpoint_every) if int(checkpoint_every) > 0 else 50, int(steps))
         q = m * (m + 1) // 2
         per_stage = nt * 3 * esz if m == 1 else nt * (m + (q + 1) // 2) * 2 * esz
         per_segment = 4 * every * per_stage

@@ -226,7 +226,8 @@ def get_initial_state(modes: Sequence[complex], p_avg: float, fwhm_s: float, rep
         p_peak = 0.939437 * p_avg / fwhm_s / rep_rate_hz
     elif pulse_form == "sech_sq":
         tau = fwhm_s / 1.7627
-        base = 1 / np.cosh(-ts / tau)
+        with np.errstate(over="ignore"):  # cosh overflows to inf far out in wide windows: sech -> 0 there
+            base = 1 / np.cosh(-ts / tau)
         p_peak = 0.881374 * p_avg / fwhm_s / rep_rate_hz
     else:
         raise ValueError(f'unsupported pulse_form={pulse_form!r}; expected "gauss" or "sech_sq"')

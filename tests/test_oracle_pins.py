@@ -542,3 +542,44 @@ def test_multimode_discrete_adjoint_shared_phase_vs_fd():
         fd = (jp - jm) / (2 * e)
         # absolute floor: FD roundoff on components far below the largest one
         assert abs(fd - grad[k]) <= 1e-6 * abs(fd) + 1e-8 * np.abs(grad).max(), (k, fd, grad[k])
+
+
+# ------------------------------------------- multimode closed forms (CW)
+
+
+@pytest.mark.parametrize("m,pattern", [(2, "quad"), (3, "phase"), (4, "quad"), (6, "phase"), (6, "quad")])
+def test_multimode_cw_closed_form(m, pattern):
+    """Cross-mode Kerr/Raman coupling of disp_mmf! (simulate_disp_mmf.jl:38-53)
+    against a closed form: a CW launch at one bin with gamma = a on (i,i,i,i)
+    and b on every arrangement of (i,i,j,j) turns every mode at the rate
+    (a + 3(M-1)b) P (all in phase) or (a + (2M-3)b) P (half the modes a
+    quarter period behind), times the shock factor of the bin, on top of the
+    dispersion phase (tests/conftest.py: cw_multimode_case).  Dropping the
+    exchange terms (i,j,i,j), (i,j,j,i) would give (a + (M-1)b) P; dropping
+    the cross-phase terms (i,i,j,j) would give (a + 2(M-1)b) P."""
+    from conftest import cw_multimode_case
+
+    D, g, hR, S, omf, u0, expected, L = cw_multimode_case(m, pattern)
+    fib = oracle.Fiber(D, g, hR, S, omf)
+    assert _rel(oracle.rk4_forward(fib, L, 1000, u0), expected) < 1e-10
+    if m <= 3:
+        assert _rel(R.rk4(u0, L, 1000, D, g, hR, S, omf), expected) < 1e-10
+    res = oracle.tsit5_forward(fib, L, u0, rtol=1e-12, atol=1e-12)
+    assert _rel(res.out, expected) < 1e-9
+
+
+def test_multimode_cw_closed_form_catches_dropped_exchange_terms():
+    """The closed form is sensitive to the cross terms: the oracle run with the
+    exchange entries gamma[i,j,i,j], gamma[i,j,j,i] (i != j) zeroed -- the
+    mistake of contracting only the cross-phase part -- misses it by far more
+    than the tolerance of the test above."""
+    from conftest import cw_multimode_case
+
+    D, g, hR, S, omf, u0, expected, L = cw_multimode_case(3, "phase")
+    bad = g.copy()
+    for i in range(3):
+        for j in range(3):
+            if i != j:
+                bad[i, j, i, j] = bad[i, j, j, i] = 0.0
+    out = oracle.rk4_forward(oracle.Fiber(D, bad, hR, S, omf), L, 400, u0)
+    assert _rel(out, expected) > 0.1
