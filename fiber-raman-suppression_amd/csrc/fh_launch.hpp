@@ -29,6 +29,13 @@ int tsit5_fields(int nt, bool fp64);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hipStream_t s);
 
+// Terminal objectives and physical gradients on device fields (fh_objective.hip).
+constexpr int kObjectiveMaxModes = 16;
+hipError_t launch_objective(bool fp64, const void* uL, const unsigned char* mask, int nt, int modes, int kind,
+                            double tau, int log_cost, void* seed, double* cost, int batch, hipStream_t s);
+hipError_t launch_physical_gradients(bool fp64, const void* lam0, const void* u0, int n, double* gphase,
+                                     double* gamp, double* edot, int batch, hipStream_t s);
+
 hipError_t launch_field_metrics(int nt, bool fp64, const void* spec, const void* tw, const double* omega, double dt,
                                 int modes, double* out, int count, hipStream_t s);
 

@@ -68,6 +68,7 @@ template <typename R> struct MmArgs {
     C2<R>* tape_h;  // [batch][regions][4 every][QC][N]
     int every, nseg;
     int ntaped;  // the last ntaped segments of each solve keep the forward sweep's tape (k_grad_multi)
+    int phase;   // gradient kernels: 1 forward sweep, 2 seed and backward sweep, 3 both (SmArgs::phase)
 };
 
 template <typename R, int N, int T, int M> struct MmTraits {
@@ -726,59 +727,65 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     const int first_kept = nseg - min(a.ntaped, nseg);
     auto seg_base = [&](int sg) { return (uint32_t)((sg >= first_kept ? sg - first_kept : a.ntaped) * 4 * a.every); };
     V v[P];
-    for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(m), st = c.fld(M + m);
+    int bad = 0;
+    double etot = 0.0, eband = 0.0;
+    if (a.phase & 1) {
+        for (int m = 0; m < M; ++m) {
+            const Fld y = c.fld(m), st = c.fld(M + m);
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
-            y.st(x, vo, CV::soff(k));
-            st.st(x, vo, CV::soff(k));
-        }
-    }
-    // forward with checkpoints of the state; kept segments tape their stages
-    const int kept_from = min(first_kept * a.every, Sb);
-    for (int step = 0; step < kept_from; ++step) {
-        if (step % a.every == 0) {
-            drain_memory();
-            for (int m = 0; m < M; ++m) {
-                const Fld st = c.fld(M + m);
-                const Fld ck{ckpt, ((uint32_t)(step / a.every) * (uint32_t)M + (uint32_t)m) * FB};
-#pragma unroll
-                for (int k = 0; k < P; ++k) ck.st(st.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+            for (int k = 0; k < P; ++k) {
+                const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+                y.st(x, vo, CV::soff(k));
+                st.st(x, vo, CV::soff(k));
             }
         }
-        const double z = step * h;
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s)
-            mm_stage<R, N, T, M, false>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, none, none, 0, v, lds);
-    }
-    for (int step = kept_from; step < Sb; ++step) {
-        const double z = step * h;
-        const int sg = step / a.every;
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s)
-            mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
-                                       seg_base(sg) + (uint32_t)(4 * (step - sg * a.every) + s), v, lds);
-    }
-    drain_memory();
-    // output and terminal seed (band energy over all modes)
-    int bad = 0;
-    const Buf out(a.out + sol * M * N, M * N * sizeof(V));
-    double etot = 0.0, eband = 0.0;
-    for (int m = 0; m < M; ++m) {
-        const Fld st = c.fld(M + m), lam = c.fld(TR::LAM + m);
+        // forward with checkpoints of the state; kept segments tape their stages
+        const int kept_from = min(first_kept * a.every, Sb);
+        for (int step = 0; step < kept_from; ++step) {
+            if (step % a.every == 0) {
+                drain_memory();
+                for (int m = 0; m < M; ++m) {
+                    const Fld st = c.fld(M + m);
+                    const Fld ck{ckpt, ((uint32_t)(step / a.every) * (uint32_t)M + (uint32_t)m) * FB};
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, cs, sn);
-            const V x = st.template ld<V>(vo, CV::soff(k));
-            const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
-            bad |= !(isfinite(o.x) && isfinite(o.y));
-            out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
-            lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
-            const double e = (double)o.x * o.x + (double)o.y * o.y;
-            etot += e;
-            if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+                    for (int k = 0; k < P; ++k) ck.st(st.template ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+                }
+            }
+            const double z = step * h;
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s)
+                mm_stage<R, N, T, M, false>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, none, none, 0, v, lds);
+        }
+        for (int step = kept_from; step < Sb; ++step) {
+            const double z = step * h;
+            const int sg = step / a.every;
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s)
+                mm_stage<R, N, T, M, true>(c, s, s == 0 ? z : (s == 3 ? z + h : z + h / 2), h, tape_u, tape_h,
+                                           seg_base(sg) + (uint32_t)(4 * (step - sg * a.every) + s), v, lds);
+        }
+        drain_memory();
+        // output and terminal seed (band energy over all modes)
+        const Buf out(a.out + sol * M * N, M * N * sizeof(V));
+        for (int m = 0; m < M; ++m) {
+            const Fld st = c.fld(M + m), lam = c.fld(TR::LAM + m);
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                R cs, sn;
+                cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), Lb, cs, sn);
+                const V x = st.template ld<V>(vo, CV::soff(k));
+                const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+                bad |= !(isfinite(o.x) && isfinite(o.y));
+                out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+                lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
+                const double e = (double)o.x * o.x + (double)o.y * o.y;
+                etot += e;
+                if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+            }
+        }
+        if (!(a.phase & 2)) {  // forward sweep only: checkpoints and kept tape stay for the backward launch
+            if (bad) atomicOr(a.status + sol, 1);
+            return;
         }
     }
     double J = 0.0, scale = 0.0;

@@ -54,6 +54,11 @@ template <typename R> struct SmArgs {
     R* tape_h;                  // [batch][regions][4*every][N]
     int every, nseg;
     int ntaped;  // the last ntaped segments keep the forward sweep's tape (k_grad_single)
+    // gradient kernels: bit 1 runs the forward sweep (checkpoints, kept tape,
+    // output), bit 2 the terminal seed and backward sweep; 3 runs both in one
+    // launch.  A backward-only launch (2) needs a caller seed and the
+    // checkpoints and kept tape of an earlier forward-only launch (1).
+    int phase;
 };
 
 template <typename R, int N, int T> struct SmTraits;
@@ -394,41 +399,47 @@ __global__ __launch_bounds__(T) void k_grad_single(SmArgs<R> a)
 
     // forward with checkpoints
     V v[P];
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-        v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
-        st.set(k, v[k]);
-    }
-    const int kept_from = min(first_kept * a.every, a.steps);
-    for (int step = 0; step < kept_from; ++step) {
-        if (step % a.every == 0) {
-            const Fld ck{ckpt, (uint32_t)(step / a.every) * (uint32_t)(N * sizeof(V))};
-#pragma unroll
-            for (int k = 0; k < P; ++k) ck.st(v[k], CV::voff(), CV::soff(k));
-        }
-        sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, none, none, 0, lds);
-    }
-    for (int step = kept_from; step < a.steps; ++step) {
-        const int sg = step / a.every;
-        sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, seg_base(sg) + 4 * (step - sg * a.every),
-                               lds);
-    }
-    // output and terminal seed
-    const Buf out(a.out + sol * N, N * sizeof(V));
     double etot = 0.0, eband = 0.0;
     int bad = 0;
+    if (a.phase & 1) {
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        R c, s;
-        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length, c, s);
-        const V x = v[k];
-        const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
-        bad |= !(isfinite(o.x) && isfinite(o.y));
-        out.st(o, CV::voff(), CV::soff(k));
-        v[k] = o;
-        const double e = (double)o.x * o.x + (double)o.y * o.y;
-        etot += e;
-        if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+        for (int k = 0; k < P; ++k) {
+            v[k] = u0.ld<V>(CV::voff(), CV::soff(k));
+            st.set(k, v[k]);
+        }
+        const int kept_from = min(first_kept * a.every, a.steps);
+        for (int step = 0; step < kept_from; ++step) {
+            if (step % a.every == 0) {
+                const Fld ck{ckpt, (uint32_t)(step / a.every) * (uint32_t)(N * sizeof(V))};
+#pragma unroll
+                for (int k = 0; k < P; ++k) ck.st(v[k], CV::voff(), CV::soff(k));
+            }
+            sm_step<R, N, T, false>(v, step, a, tb, st, ac, ut, none, none, 0, lds);
+        }
+        for (int step = kept_from; step < a.steps; ++step) {
+            const int sg = step / a.every;
+            sm_step<R, N, T, true>(v, step, a, tb, st, ac, ut, tape_u, tape_h, seg_base(sg) + 4 * (step - sg * a.every),
+                                   lds);
+        }
+        // output and terminal seed
+        const Buf out(a.out + sol * N, N * sizeof(V));
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            R c, s;
+            cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length, c, s);
+            const V x = v[k];
+            const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
+            bad |= !(isfinite(o.x) && isfinite(o.y));
+            out.st(o, CV::voff(), CV::soff(k));
+            v[k] = o;
+            const double e = (double)o.x * o.x + (double)o.y * o.y;
+            etot += e;
+            if (!a.seed && a.mask[threadIdx.x + T * k]) eband += e;
+        }
+        if (!(a.phase & 2)) {  // forward sweep only: checkpoints and kept tape stay for a later backward launch
+            if (bad) atomicOr(a.status + sol, 1);
+            return;
+        }
     }
     if (!a.seed) {
         etot = block_sum<T>(etot, lds);

@@ -429,30 +429,34 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
     const Buf tape_h(sa.tape_h + sol * tslots * N, (uint32_t)(tslots * N * sizeof(R)));
     const Buf u0(sa.u0 + sol * N, N * sizeof(V));
     const uint32_t vo = CV::voff();
-    {
-        const Fld y = fld(LY::Y);
-#pragma unroll
-        for (int k = 0; k < P; ++k) y.st(u0.ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
-    }
-    drain_memory();
-    int ys;
+    int ys = LY::Y, bad = 0;
     long long nacc, nrej, nfe;
-    int bad = ts_forward<R, N, T, true>(a, tb, base, ut, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, lds);
-    if (threadIdx.x == 0) {
-        a.stats[sol * 3 + 0] = nacc;
-        a.stats[sol * 3 + 1] = nrej;
-        a.stats[sol * 3 + 2] = nfe;
-    }
-    if (bad) {
-        if (threadIdx.x == 0) atomicOr(sa.status + sol, bad);
-        return;  // workgroup-uniform
+    if (sa.phase & 1) {
+        {
+            const Fld y = fld(LY::Y);
+#pragma unroll
+            for (int k = 0; k < P; ++k) y.st(u0.ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+        }
+        drain_memory();
+        bad = ts_forward<R, N, T, true>(a, tb, base, ut, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, lds);
+        if (threadIdx.x == 0) {
+            a.stats[sol * 3 + 0] = nacc;
+            a.stats[sol * 3 + 1] = nrej;
+            a.stats[sol * 3 + 2] = nfe;
+        }
+        if (bad) {
+            if (threadIdx.x == 0) atomicOr(sa.status + sol, bad);
+            return;  // workgroup-uniform
+        }
+    } else {
+        nacc = a.stats[sol * 3 + 0];  // accepted steps of the forward-only launch
     }
     drain_memory();
     // output and terminal seed
     V v[P];
     const Buf out(sa.out + sol * N, N * sizeof(V));
     double etot = 0.0, eband = 0.0;
-    {
+    if (sa.phase & 1) {
         const Fld y = fld(ys);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
@@ -466,6 +470,10 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
             const double e = (double)o.x * o.x + (double)o.y * o.y;
             etot += e;
             if (!sa.seed && sa.mask[threadIdx.x + T * k]) eband += e;
+        }
+        if (!(sa.phase & 2)) {  // forward sweep only: checkpoints, kept tape and steps stay for the backward launch
+            if (bad) atomicOr(sa.status + sol, bad);
+            return;
         }
     }
     if (!sa.seed) {
@@ -623,7 +631,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5(TsArgs<R> a)
             gr.st(g, CD::voff(), CD::soff(k));
         }
     }
-    if (bad && threadIdx.x == 0) atomicOr(sa.status + sol, bad);
+    if (bad) atomicOr(sa.status + sol, bad);
 }
 
 }  // namespace fh

@@ -388,43 +388,53 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     const uint32_t vo = CV::voff();
     const double L = ma.lengths ? ma.lengths[sol] : ma.length;
     V v[P];
-#pragma unroll 1
-    for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(LF::Y + m);
-#pragma unroll
-        for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
-    }
-    int ys;
+    int ys = LF::Y, bad = 0;
     long long nacc, nrej, nfe;
-    int bad = mt_forward<R, N, T, M, true>(a, c, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, v, lds);
-    if (threadIdx.x == 0) {
-        a.stats[sol * 3 + 0] = nacc;
-        a.stats[sol * 3 + 1] = nrej;
-        a.stats[sol * 3 + 2] = nfe;
-    }
-    if (bad) {
-        if (threadIdx.x == 0) atomicOr(ma.status + sol, bad);
-        return;  // workgroup-uniform
+    if (ma.phase & 1) {
+#pragma unroll 1
+        for (int m = 0; m < M; ++m) {
+            const Fld y = c.fld(LF::Y + m);
+#pragma unroll
+            for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
+        }
+        bad = mt_forward<R, N, T, M, true>(a, c, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, v, lds);
+        if (threadIdx.x == 0) {
+            a.stats[sol * 3 + 0] = nacc;
+            a.stats[sol * 3 + 1] = nrej;
+            a.stats[sol * 3 + 2] = nfe;
+        }
+        if (bad) {
+            if (threadIdx.x == 0) atomicOr(ma.status + sol, bad);
+            return;  // workgroup-uniform
+        }
+    } else {
+        nacc = a.stats[sol * 3 + 0];  // accepted steps of the forward-only launch
     }
     drain_memory();
     // output and terminal seed (band energy over all modes)
     const Buf out(ma.out + sol * M * N, M * N * sizeof(V));
     double etot = 0.0, eband = 0.0;
+    if (ma.phase & 1) {
 #pragma unroll 1
-    for (int m = 0; m < M; ++m) {
-        const Fld y = c.fld(ys + m), lam = c.fld(TR::LAM + m);
+        for (int m = 0; m < M; ++m) {
+            const Fld y = c.fld(ys + m), lam = c.fld(TR::LAM + m);
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), L, cs, sn);
-            const V x = y.template ld<V>(vo, CV::soff(k));
-            const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
-            bad |= !(isfinite(o.x) && isfinite(o.y));
-            out.st(o, vo, (uint32_t)m * FB + CV::soff(k));
-            lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
-            const double e = (double)o.x * o.x + (double)o.y * o.y;
-            etot += e;
-            if (!ma.seed && ma.mask[threadIdx.x + T * k]) eband += e;
+            for (int k = 0; k < P; ++k) {
+                R cs, sn;
+                cis_arg(disp.ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), L, cs, sn);
+                const V x = y.template ld<V>(vo, CV::soff(k));
+                const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+                bad |= !(isfinite(o.x) && isfinite(o.y));
+                out.st(o, vo, (uint32_t)m * FB + CV::soff(k));
+                lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
+                const double e = (double)o.x * o.x + (double)o.y * o.y;
+                etot += e;
+                if (!ma.seed && ma.mask[threadIdx.x + T * k]) eband += e;
+            }
+        }
+        if (!(ma.phase & 2)) {  // forward sweep only: checkpoints, kept tape and steps stay for the backward launch
+            if (bad) atomicOr(ma.status + sol, bad);
+            return;
         }
     }
     double J = 0.0, scale = 0.0;

@@ -60,6 +60,22 @@ struct fh_plan_s {
     bool gamma_full_sym = true; // gamma invariant under i<->j, j<->k, k<->l (sensitivity_disp_mmf.jl:103-112)
     int max_steps = 0;
     DevBuf hs, zs, sstats;      // adaptive: step sizes, positions [max_batch][max_steps]; counters [max_batch][3]
+    size_t grad_work_per_solve = 0;  // multimode gradient slice (complex entries per solve)
+    size_t ckpt_per_solve = 0, tape_u_per_solve = 0, tape_h_per_solve = 0;  // bytes
+    std::string grad_error;          // why this plan offers no gradient (empty: offered)
+    fh_tape spare = nullptr;         // storage of a freed tape, reused by the next fh_forward_tape
+    int64_t gradient_bytes = 0;      // plan bytes once the adjoint buffers exist (layout)
+    bool grad_work = false;          // multimode: the work slice has the gradient kernels' size
+};
+
+// A kept forward solve (fh_forward_tape): the shaped launch, the output, the
+// checkpoints and kept stage tape, and the accepted steps of adaptive plans.
+struct fh_tape_s {
+    fh_plan plan = nullptr;
+    int batch = 0, capacity = 0;  // solves kept; solves the buffers hold
+    bool swept = false;           // a single-mode backward sweep has consumed the kept tape
+    DevBuf u0, uL, ckpt, tape_u, tape_h, hs, zs, sstats, status, seed, lam0, kgrad, gphase, gamp, edot, cost;
+    int64_t bytes = 0;
 };
 
 namespace {
@@ -175,6 +191,16 @@ template <typename R, int N> struct OpTwMulti {
     }
     static int unsupported() { return 0; }
 };
+template <typename R, int N> struct OpShape {
+    // workgroup size and twiddle-table entries of the single-mode or multimode kernels
+    static std::pair<int, int> run(bool multi)
+    {
+        constexpr int TS = fh::SingleShape<R, N>::T, TM = fh::MultiShape<R, N>::T;
+        return multi ? std::make_pair(TM, (int)fh::FftPlan<N, TM>::TW_SIZE)
+                     : std::make_pair(TS, (int)fh::FftPlan<N, TS>::TW_SIZE);
+    }
+    static std::pair<int, int> unsupported() { return {0, 0}; }
+};
 template <typename R, int N> struct OpGrad {
     static hipError_t run(const fh::SmArgs<R>& a, int batch, hipStream_t s) { return GradKernel<R, N>::launch(a, batch, s); }
     static hipError_t unsupported() { return hipErrorInvalidValue; }
@@ -256,6 +282,10 @@ int check_desc(const fh_problem_desc* d)
         return fail(FH_EUNSUPPORTED, "dispersion phase |D|*L exceeds " + std::to_string(FH_MAX_PHASE) + " rad");
     if (d->checkpoint_every < 0) return fail(FH_EINVAL, "checkpoint_every must be >= 0");
     if (d->tape_segments < 0) return fail(FH_EINVAL, "tape_segments must be >= 0");
+    if (d->objective != FH_OBJ_BAND_SUM && d->objective != FH_OBJ_FUNDAMENTAL && d->objective != FH_OBJ_WORST_MODE)
+        return fail(FH_EINVAL, "unknown objective");
+    if (!(std::isfinite(d->objective_tau) && d->objective_tau >= 0))
+        return fail(FH_EINVAL, "objective_tau must be finite and >= 0 (0 = 50)");
     if (d->modes == 1) {
         const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
         if (d->nt < 64 || d->nt > nmax)
@@ -267,13 +297,135 @@ int check_desc(const fh_problem_desc* d)
     return FH_OK;
 }
 
-template <typename R> int upload_tables(fh_plan p)
+// Everything a plan allocates, derived from the descriptor alone (no device
+// needed): fh_plan_create allocates these sizes, the first gradient call the
+// adjoint part, and fh_plan_check reports them.
+struct PlanLayout {
+    int threads = 0, tw_entries = 0;
+    bool adaptive = false;
+    int every = 50, nseg = 0, ntaped = 0, max_steps = 0, q = 0;
+    bool gamma_full_sym = true;
+    std::vector<std::pair<int, int>> outp, inp;  // multimode output / input pairs (i, j)
+    std::vector<int> qidx;                        // [M][M] output-pair index
+    size_t work_per_solve = 0;       // complex entries of the forward kernels' slice
+    size_t grad_work_per_solve = 0;  // multimode gradient slice (the forward slice serves single mode)
+    size_t ckpt_per_solve = 0, tape_u_per_solve = 0, tape_h_per_solve = 0;  // bytes
+    std::string grad_error;          // why a gradient is not offered (empty: offered)
+    int64_t forward_bytes = 0, gradient_bytes = 0;  // device bytes of the plan, without / with adjoint buffers
+};
+
+size_t alloc_size(size_t bytes) { return bytes == 0 ? 16 : bytes; }  // dev_alloc's rounding
+
+template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout& L)
+{
+    const size_t nt = (size_t)d.nt, M = (size_t)d.modes, mb = (size_t)d.max_batch, V = 2 * sizeof(R);
+    const auto shape = with_size<R, OpShape>(d.nt, d.modes > 1);
+    if (shape.first == 0) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(d.nt));
+    L.threads = shape.first;
+    L.tw_entries = shape.second;
+    L.adaptive = d.scheme == FH_SCHEME_TSIT5;
+    L.every = d.checkpoint_every > 0 ? d.checkpoint_every : 50;
+    if (L.adaptive) {
+        L.max_steps = d.max_steps > 0 ? d.max_steps : 10000;
+        if (L.every > L.max_steps) L.every = L.max_steps;
+        L.nseg = (L.max_steps + L.every - 1) / L.every;
+        L.work_per_solve = M == 1 ? (size_t)fh::tsit5_fields(d.nt, sizeof(R) == 8) * nt
+                                  : (size_t)fh::tsit5_multi_fields(d.modes) * nt;
+    } else {
+        if (L.every > d.steps) L.every = d.steps;
+        L.nseg = (d.steps + L.every - 1) / L.every;
+        L.work_per_solve = M == 1 ? with_size<R, OpWork>(d.nt) : (size_t)fh::multi_fields(d.modes) * nt;
+    }
+    L.ntaped = std::min(d.tape_segments, L.nseg);
+    if (M > 1) {
+        const int m = d.modes;
+        auto G = [&](int i, int j, int k, int l) { return d.gamma[i + m * (j + m * (k + m * l))]; };
+        double gmax = 0;
+        for (size_t i = 0; i < M * M * M * M; ++i) gmax = std::max(gmax, std::fabs(d.gamma[i]));
+        bool sym = true;
+        for (int i = 0; i < m && sym; ++i)
+            for (int j = 0; j < m && sym; ++j)
+                for (int k = 0; k < m && sym; ++k)
+                    for (int l = 0; l < m; ++l)
+                        if (std::fabs(G(i, j, k, l) - G(j, i, k, l)) > 1e-12 * std::max(1.0, gmax)) {
+                            sym = false;
+                            break;
+                        }
+        for (int i = 0; i < m; ++i)
+            for (int j = i; j < m; ++j) L.inp.push_back({i, j});
+        L.qidx.assign(M * M, 0);
+        for (int i = 0; i < m; ++i)
+            for (int j = sym ? i : 0; j < m; ++j) {
+                L.qidx[i * m + j] = (int)L.outp.size();
+                if (sym) L.qidx[j * m + i] = (int)L.outp.size();
+                L.outp.push_back({i, j});
+            }
+        L.q = (int)L.outp.size();
+        const double tol = 1e-12 * std::max(1.0, gmax);
+        auto close = [&](double x, double y) { return std::fabs(x - y) <= tol + 1e-12 * std::fabs(y); };
+        for (int i = 0; i < m && L.gamma_full_sym; ++i)
+            for (int j = 0; j < m && L.gamma_full_sym; ++j)
+                for (int k = 0; k < m && L.gamma_full_sym; ++k)
+                    for (int l = 0; l < m; ++l) {
+                        const double g = G(i, j, k, l);
+                        if (!close(g, G(j, i, k, l)) || !close(g, G(i, k, j, l)) || !close(g, G(i, j, l, k))) {
+                            L.gamma_full_sym = false;
+                            break;
+                        }
+                    }
+    }
+    // forward: operators, twiddles, mask, reduced gamma, slice, status, fields, step records
+    const size_t field = V * nt * M * mb;
+    size_t fwd = alloc_size(8 * nt * M) + alloc_size(sizeof(R) * 2 * nt) + alloc_size(sizeof(R) * nt) +
+                 alloc_size(sizeof(R) * 2 * (size_t)L.tw_entries);
+    if (d.band_mask) fwd += alloc_size(nt);
+    if (M > 1) fwd += alloc_size(sizeof(R) * L.outp.size() * L.inp.size()) + alloc_size(sizeof(int) * M * M);
+    const size_t work = alloc_size(V * L.work_per_solve * mb);
+    fwd += work + alloc_size(sizeof(int) * mb) + 2 * alloc_size(field);
+    if (L.adaptive) fwd += 2 * alloc_size(sizeof(double) * mb * (size_t)L.max_steps) + alloc_size(sizeof(long long) * 3 * mb);
+    L.forward_bytes = (int64_t)fwd;
+    // gradient: checkpoints, stage tape (one region per kept segment plus the
+    // shared region of recomputed segments: fixed step only when some
+    // segments are recomputed; adaptive always, as the segment count is known
+    // only at the end of a solve), adjoint fields
+    if (M > 1)
+        L.grad_work_per_solve =
+            (size_t)(L.adaptive ? fh::tsit5_multi_grad_fields(d.modes) : fh::multi_grad_fields(d.modes)) * nt;
+    L.ckpt_per_solve = V * nt * M * (size_t)L.nseg;
+    const size_t stages = L.adaptive ? 6 : 4;
+    const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((L.q + 1) / 2);
+    const size_t regions = (size_t)L.ntaped + ((L.adaptive || L.ntaped < L.nseg) ? 1 : 0);
+    L.tape_u_per_solve = V * nt * M * stages * (size_t)L.every * regions;
+    L.tape_h_per_solve = hbytes * stages * (size_t)L.every * regions;
+    if (L.ckpt_per_solve > 0xffffffffull)
+        L.grad_error = "the checkpoints of one solve exceed 4 GiB (" + std::to_string(L.ckpt_per_solve) +
+                       " bytes); raise checkpoint_every or lower steps / max_steps";
+    else if (L.tape_u_per_solve > 0xffffffffull || L.tape_h_per_solve > 0xffffffffull)
+        L.grad_error = "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every";
+    else if (M > 1 && V * L.grad_work_per_solve > 0xffffffffull)
+        L.grad_error = "the gradient work slice of one solve exceeds 4 GiB";
+    size_t grad = fwd;
+    if (M > 1) grad = grad - work + alloc_size(V * L.grad_work_per_solve * mb);
+    grad += alloc_size(L.ckpt_per_solve * mb) + alloc_size(L.tape_u_per_solve * mb) +
+            alloc_size(L.tape_h_per_solve * mb) + 2 * alloc_size(field) + alloc_size(sizeof(double) * nt * mb) +
+            alloc_size(sizeof(double) * mb);
+    L.gradient_bytes = (int64_t)grad;
+    return FH_OK;
+}
+
+int compute_layout(const fh_problem_desc& d, PlanLayout& L)
+{
+    return d.precision == FH_PREC_FP64 ? compute_layout_t<double>(d, L) : compute_layout_t<float>(d, L);
+}
+
+template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
 {
     const fh_problem_desc& d = p->d;
     const int nt = d.nt;
     std::vector<std::complex<long double>> tw;
     const int T = d.modes == 1 ? with_size<R, OpTw>(nt, tw) : with_size<R, OpTwMulti>(nt, tw);
-    if (T == 0) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(nt));
+    if (T != L.threads || (int)tw.size() != L.tw_entries)
+        return fail(FH_EINVAL, "internal: kernel shape differs from the plan layout");
     p->threads = T;
     std::vector<R> twr(2 * tw.size()), hn(2 * (size_t)nt), shn(nt);
     for (size_t i = 0; i < tw.size(); ++i) {
@@ -303,81 +455,36 @@ template <typename R> int upload_tables(fh_plan p)
         FH_HIP(hipMemcpy(p->mask.p, d.band_mask, (size_t)nt, hipMemcpyHostToDevice));
         p->has_mask = true;
     }
-    p->every = d.checkpoint_every > 0 ? d.checkpoint_every : 50;
-    if (p->adaptive) {
-        p->max_steps = d.max_steps > 0 ? d.max_steps : 10000;
-        if (p->every > p->max_steps) p->every = p->max_steps;
-        p->nseg = (p->max_steps + p->every - 1) / p->every;
-        p->ntaped = std::min(d.tape_segments, p->nseg);
-        p->work_per_solve = d.modes == 1 ? (size_t)fh::tsit5_fields(nt, sizeof(R) == 8) * nt
-                                         : (size_t)fh::tsit5_multi_fields(d.modes) * nt;
-    } else {
-        if (p->every > d.steps) p->every = d.steps;
-        p->nseg = (d.steps + p->every - 1) / p->every;
-        p->ntaped = std::min(d.tape_segments, p->nseg);
-        p->work_per_solve = d.modes == 1 ? with_size<R, OpWork>(nt) : (size_t)fh::multi_fields(d.modes) * nt;
-    }
+    p->every = L.every;
+    p->nseg = L.nseg;
+    p->ntaped = L.ntaped;
+    p->max_steps = L.max_steps;
+    p->work_per_solve = L.work_per_solve;
+    p->grad_work_per_solve = L.grad_work_per_solve;
+    p->ckpt_per_solve = L.ckpt_per_solve;
+    p->tape_u_per_solve = L.tape_u_per_solve;
+    p->tape_h_per_solve = L.tape_h_per_solve;
+    p->grad_error = L.grad_error;
+    p->q = L.q;
+    p->gamma_full_sym = L.gamma_full_sym;
     if (d.modes > 1) {
         const int M = d.modes;
         auto G = [&](int i, int j, int k, int l) { return d.gamma[i + M * (j + M * (k + M * l))]; };
-        const double gmax = [&] {
-            double g = 0;
-            for (int i = 0; i < M * M * M * M; ++i) g = std::max(g, std::fabs(d.gamma[i]));
-            return g;
-        }();
-        bool sym = true;
-        for (int i = 0; i < M && sym; ++i)
-            for (int j = 0; j < M && sym; ++j)
-                for (int k = 0; k < M && sym; ++k)
-                    for (int l = 0; l < M; ++l)
-                        if (std::fabs(G(i, j, k, l) - G(j, i, k, l)) > 1e-12 * std::max(1.0, gmax)) {
-                            sym = false;
-                            break;
-                        }
-        std::vector<std::pair<int, int>> outp, inp;
-        for (int i = 0; i < M; ++i)
-            for (int j = i; j < M; ++j) inp.push_back({i, j});
-        std::vector<int> qidx(M * M);
-        if (sym) {
-            for (int i = 0; i < M; ++i)
-                for (int j = i; j < M; ++j) {
-                    qidx[i * M + j] = qidx[j * M + i] = (int)outp.size();
-                    outp.push_back({i, j});
-                }
-        } else {
-            for (int i = 0; i < M; ++i)
-                for (int j = 0; j < M; ++j) {
-                    qidx[i * M + j] = (int)outp.size();
-                    outp.push_back({i, j});
-                }
-        }
-        p->q = (int)outp.size();
+        const bool sym = L.q != M * M;
         for (int i = 0; i < M; ++i)
             for (int j = 0; j < M; ++j)
-                if (qidx[i * M + j] != fh::pair_index(M, i, j, sym))
+                if (L.qidx[i * M + j] != fh::pair_index(M, i, j, sym))
                     return fail(FH_EINVAL, "internal: output-pair order differs from the kernels' pair_index");
-        const double tol = 1e-12 * std::max(1.0, gmax);
-        auto close = [&](double x, double y) { return std::fabs(x - y) <= tol + 1e-12 * std::fabs(y); };
-        for (int i = 0; i < M && p->gamma_full_sym; ++i)
-            for (int j = 0; j < M && p->gamma_full_sym; ++j)
-                for (int k = 0; k < M && p->gamma_full_sym; ++k)
-                    for (int l = 0; l < M; ++l) {
-                        const double g = G(i, j, k, l);
-                        if (!close(g, G(j, i, k, l)) || !close(g, G(i, k, j, l)) || !close(g, G(i, j, l, k))) {
-                            p->gamma_full_sym = false;
-                            break;
-                        }
-                    }
-        std::vector<R> gr(outp.size() * inp.size());
-        for (size_t qq = 0; qq < outp.size(); ++qq)
-            for (size_t pp = 0; pp < inp.size(); ++pp) {
-                const int i = outp[qq].first, j = outp[qq].second, k = inp[pp].first, l = inp[pp].second;
-                gr[qq * inp.size() + pp] = (R)(k == l ? G(i, j, k, k) : G(i, j, k, l) + G(i, j, l, k));
+        std::vector<R> gr(L.outp.size() * L.inp.size());
+        for (size_t qq = 0; qq < L.outp.size(); ++qq)
+            for (size_t pp = 0; pp < L.inp.size(); ++pp) {
+                const int i = L.outp[qq].first, j = L.outp[qq].second, k = L.inp[pp].first, l = L.inp[pp].second;
+                gr[qq * L.inp.size() + pp] = (R)(k == l ? G(i, j, k, k) : G(i, j, k, l) + G(i, j, l, k));
             }
         if ((rc = dev_alloc(p, p->gr, sizeof(R) * gr.size()))) return rc;
-        if ((rc = dev_alloc(p, p->qidx, sizeof(int) * qidx.size()))) return rc;
+        if ((rc = dev_alloc(p, p->qidx, sizeof(int) * L.qidx.size()))) return rc;
         FH_HIP(hipMemcpy(p->gr.p, gr.data(), p->gr.bytes, hipMemcpyHostToDevice));
-        FH_HIP(hipMemcpy(p->qidx.p, qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice));
+        FH_HIP(hipMemcpy(p->qidx.p, L.qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice));
     }
     const size_t mb = (size_t)d.max_batch;
     if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
@@ -390,6 +497,7 @@ template <typename R> int upload_tables(fh_plan p)
         if ((rc = dev_alloc(p, p->zs, sizeof(double) * mb * (size_t)p->max_steps))) return rc;
         if ((rc = dev_alloc(p, p->sstats, sizeof(long long) * 3 * mb))) return rc;
     }
+    if (p->total_bytes != L.forward_bytes) return fail(FH_EINVAL, "internal: plan bytes differ from the plan layout");
     return FH_OK;
 }
 
@@ -426,44 +534,61 @@ template <typename R> fh::SmArgs<R> base_args(fh_plan p)
     a.ntaped = p->ntaped;
     a.log_cost = d.log_cost;
     a.mask = (const unsigned char*)p->mask.p;
+    a.phase = 3;
     return a;
+}
+
+// Multimode gradient kernels keep the adjoint fields in the work slice too,
+// so the slice grows to their size before the first gradient launch.
+template <typename R> int ensure_grad_work(fh_plan p)
+{
+    if (!p->grad_error.empty()) return fail(FH_EINVAL, p->grad_error);
+    if (p->grad_work || p->d.modes == 1) return FH_OK;
+    total_release(p, p->work);
+    int rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->grad_work_per_solve * (size_t)p->d.max_batch);
+    if (rc) return rc;
+    p->grad_work = true;
+    return FH_OK;
 }
 
 template <typename R> int ensure_adjoint_buffers(fh_plan p)
 {
     if (p->ckpt.p) return FH_OK;
     const size_t nt = (size_t)p->d.nt, mb = (size_t)p->d.max_batch, V = 2 * sizeof(R), M = (size_t)p->d.modes;
-    int rc;
-    if (M > 1) {  // the gradient kernel's slice holds the adjoint fields too
-        total_release(p, p->work);
-        const int f = p->adaptive ? fh::tsit5_multi_grad_fields((int)M) : fh::multi_grad_fields((int)M);
-        if ((rc = dev_alloc(p, p->work, V * (size_t)f * nt * mb))) return rc;
-    }
-    if ((rc = dev_alloc(p, p->ckpt, V * nt * M * (size_t)p->nseg * mb))) return rc;
-    const size_t stages = p->adaptive ? 6 : 4;
-    // tape per stage: u of every mode; h as one real field (single mode) or ceil(Q/2) packed pairs
-    const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((p->q + 1) / 2);
-    // single mode: one region per kept segment plus the shared recomputation
-    // region (fixed step: only when some segments are recomputed; adaptive:
-    // always, since the segment count is known only at the end of a solve)
-    const size_t regions = (size_t)p->ntaped + ((p->adaptive || p->ntaped < p->nseg) ? 1 : 0);
-    const size_t solve_tape = V * nt * M * stages * (size_t)p->every * regions;
-    const size_t solve_tape_h = hbytes * stages * (size_t)p->every * regions;
-    if (solve_tape > 0xffffffffull || solve_tape_h > 0xffffffffull)
-        return fail(FH_EINVAL, "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every");
-    if ((rc = dev_alloc(p, p->tape_u, solve_tape * mb))) return rc;
-    if ((rc = dev_alloc(p, p->tape_h, solve_tape_h * mb))) return rc;
+    int rc = ensure_grad_work<R>(p);
+    if (rc) return rc;
+    if ((rc = dev_alloc(p, p->ckpt, p->ckpt_per_solve * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_u, p->tape_u_per_solve * mb))) return rc;
+    if ((rc = dev_alloc(p, p->tape_h, p->tape_h_per_solve * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_lam0, V * nt * M * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_grad, sizeof(double) * nt * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_cost, sizeof(double) * mb))) return rc;
     if ((rc = dev_alloc(p, p->g_seed, V * nt * M * mb))) return rc;
+    if (p->total_bytes != p->gradient_bytes)
+        return fail(FH_EINVAL, "internal: gradient plan bytes differ from the plan layout");
     return FH_OK;
 }
 
+// Device buffers of one gradient launch beyond its inputs and outputs: the
+// plan's own for fh_gradient*, a tape's for fh_forward_tape / fh_adjoint.
+struct GradBufs {
+    void *ckpt, *tape_u, *tape_h;
+    double *hs, *zs;
+    long long* stats;
+    int* status;
+    int phase;  // SmArgs::phase
+};
+
+GradBufs plan_grad_bufs(fh_plan p)
+{
+    return GradBufs{p->ckpt.p, p->tape_u.p, p->tape_h.p, (double*)p->hs.p, (double*)p->zs.p,
+                    (long long*)p->sstats.p, (int*)p->status.p, 3};
+}
+
 template <typename R>
-int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
-                    double* d_grad, double* d_cost, hipStream_t s, const double* d_lengths = nullptr,
-                    const int* d_nsteps = nullptr)
+int launch_gradient_with(fh_plan p, const GradBufs& gb, int batch, const void* d_u0, const void* d_seed,
+                         void* d_uL, void* d_lam0, double* d_grad, double* d_cost, hipStream_t s,
+                         const double* d_lengths = nullptr, const int* d_nsteps = nullptr)
 {
     if (!d_seed && !p->has_mask)
         return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
@@ -473,8 +598,6 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
         if (!p->adaptive && !d_lengths != !d_nsteps)
             return fail(FH_EINVAL, "multimode per-solve lengths need per-solve step counts");
     }
-    int rc = ensure_adjoint_buffers<R>(p);
-    if (rc) return rc;
     if (p->d.modes > 1) {
         const fh_problem_desc& d = p->d;
         fh::MmArgs<R> m{};
@@ -487,7 +610,7 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
         m.tw = (const fh::C2<R>*)p->tw.p;
         m.gr = (const R*)p->gr.p;
         m.qidx = (const int*)p->qidx.p;
-        m.status = (int*)p->status.p;
+        m.status = gb.status;
         m.one_m_fr = (R)d.one_m_fr;
         m.h = d.length_m / d.steps;
         m.length = d.length_m;
@@ -499,15 +622,16 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
         m.lam0 = (fh::C2<R>*)d_lam0;
         m.grad = d_grad;
         m.cost = d_cost;
-        m.ckpt = (fh::C2<R>*)p->ckpt.p;
-        m.tape_u = (fh::C2<R>*)p->tape_u.p;
-        m.tape_h = (fh::C2<R>*)p->tape_h.p;
+        m.ckpt = (fh::C2<R>*)gb.ckpt;
+        m.tape_u = (fh::C2<R>*)gb.tape_u;
+        m.tape_h = (fh::C2<R>*)gb.tape_h;
+        m.phase = gb.phase;
         m.every = p->every;
         m.nseg = p->nseg;
         m.ntaped = p->ntaped;
         m.lengths = d_lengths;
         m.nsteps = d_nsteps;  // each at most the plan's steps (checkpoint capacity)
-        FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+        if (gb.phase & 1) FH_HIP(hipMemsetAsync(gb.status, 0, sizeof(int) * (size_t)batch, s));
         hipError_t e;
         if (p->adaptive) {
             m.h = 0.0;
@@ -518,9 +642,9 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
             t.rtol = d.rtol;
             t.atol = d.atol;
             t.max_steps = p->max_steps;
-            t.hs = (double*)p->hs.p;
-            t.zs = (double*)p->zs.p;
-            t.stats = (long long*)p->sstats.p;
+            t.hs = gb.hs;
+            t.zs = gb.zs;
+            t.stats = gb.stats;
             e = fh::launch_tsit5_multi(d.nt, d.modes, true, t, batch, s);
         } else {
             e = fh::launch_multi_grad(d.nt, d.modes, m, batch, s);
@@ -535,17 +659,52 @@ int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, 
     a.lam0 = (fh::C2<R>*)d_lam0;
     a.grad = d_grad;
     a.cost = d_cost;
-    a.ckpt = (fh::C2<R>*)p->ckpt.p;
-    a.tape_u = (fh::C2<R>*)p->tape_u.p;
-    a.tape_h = (R*)p->tape_h.p;
-    FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    a.ckpt = (fh::C2<R>*)gb.ckpt;
+    a.tape_u = (fh::C2<R>*)gb.tape_u;
+    a.tape_h = (R*)gb.tape_h;
+    a.status = gb.status;
+    a.phase = gb.phase;
     if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
+    if (gb.phase & 1) FH_HIP(hipMemsetAsync(gb.status, 0, sizeof(int) * (size_t)batch, s));
     fh::TsArgs<R> ta = ts_args<R>(p, a);
+    ta.hs = gb.hs;
+    ta.zs = gb.zs;
+    ta.stats = gb.stats;
     ta.lengths = d_lengths;
     hipError_t e = p->adaptive ? fh::launch_tsit5(p->d.nt, true, ta, batch, s)
                                : with_size<R, OpGrad>(p->d.nt, a, batch, s);
     if (e != hipSuccess) return fail(FH_EDEVICE, std::string("gradient launch: ") + hipGetErrorString(e));
     return FH_OK;
+}
+
+template <typename R>
+int launch_gradient(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
+                    double* d_grad, double* d_cost, hipStream_t s, const double* d_lengths = nullptr,
+                    const int* d_nsteps = nullptr)
+{
+    if (!d_seed && !p->has_mask)
+        return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
+    int rc = ensure_adjoint_buffers<R>(p);
+    if (rc) return rc;
+    GradBufs gb = plan_grad_bufs(p);
+    if (d_seed || p->d.modes == 1 || p->d.objective == FH_OBJ_BAND_SUM)
+        return launch_gradient_with<R>(p, gb, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s, d_lengths,
+                                       d_nsteps);
+    // another multimode objective: forward sweep, its seed on the device from
+    // the output, backward sweep
+    if ((p->d.modes > 1 && !p->gamma_full_sym))
+        return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
+    gb.phase = 1;
+    if ((rc = launch_gradient_with<R>(p, gb, batch, d_u0, p->g_seed.p, d_uL, d_lam0, d_grad, d_cost, s, d_lengths,
+                                      d_nsteps)))
+        return rc;
+    const double tau = p->d.objective_tau > 0 ? p->d.objective_tau : 50.0;
+    hipError_t e = fh::launch_objective(sizeof(R) == 8, d_uL, (const unsigned char*)p->mask.p, p->d.nt, p->d.modes,
+                                        p->d.objective, tau, p->d.log_cost, p->g_seed.p, d_cost, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("objective launch: ") + hipGetErrorString(e));
+    gb.phase = 2;
+    return launch_gradient_with<R>(p, gb, batch, d_u0, p->g_seed.p, d_uL, d_lam0, d_grad, d_cost, s, d_lengths,
+                                   d_nsteps);
 }
 
 template <typename R>
@@ -660,6 +819,123 @@ int finish_call(fh_plan p, int batch, const std::vector<int>& status, fh_stats* 
     return FH_OK;
 }
 
+// ------------------------------------------------------------ tapes
+
+void release_tape_storage(fh_tape t)
+{
+    for (DevBuf* b : {&t->u0, &t->uL, &t->ckpt, &t->tape_u, &t->tape_h, &t->hs, &t->zs, &t->sstats, &t->status,
+                      &t->seed, &t->lam0, &t->kgrad, &t->gphase, &t->gamp, &t->edot, &t->cost})
+        dev_free(*b);
+    delete t;
+}
+
+int tape_alloc(fh_tape t, DevBuf& b, size_t bytes)
+{
+    bytes = alloc_size(bytes);
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? FH_ENOMEM : FH_EDEVICE,
+                    std::string("hipMalloc(") + std::to_string(bytes) + ") for a tape: " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    t->bytes += (int64_t)bytes;
+    return FH_OK;
+}
+
+// Storage for `batch` kept solves: the plan's spare when it is large enough.
+int new_tape(fh_plan p, int batch, fh_tape* out)
+{
+    if (p->spare && p->spare->capacity >= batch) {
+        *out = p->spare;
+        p->spare = nullptr;
+    } else {
+        if (p->spare) release_tape_storage(p->spare);
+        p->spare = nullptr;
+        fh_tape t = new fh_tape_s;
+        const size_t nt = (size_t)p->d.nt, M = (size_t)p->d.modes, B = (size_t)batch;
+        const size_t V = p->d.precision == FH_PREC_FP64 ? 16 : 8, field = V * nt * M * B;
+        const size_t steps = p->adaptive ? (size_t)p->max_steps : 0;
+        int rc = FH_OK;
+        for (auto& pr : std::vector<std::pair<DevBuf*, size_t>>{
+                 {&t->u0, field}, {&t->uL, field}, {&t->seed, field}, {&t->lam0, field},
+                 {&t->ckpt, p->ckpt_per_solve * B}, {&t->tape_u, p->tape_u_per_solve * B},
+                 {&t->tape_h, p->tape_h_per_solve * B}, {&t->hs, sizeof(double) * steps * B},
+                 {&t->zs, sizeof(double) * steps * B}, {&t->sstats, sizeof(long long) * 3 * B},
+                 {&t->status, sizeof(int) * B}, {&t->kgrad, sizeof(double) * nt * B},
+                 {&t->gphase, sizeof(double) * nt * M * B}, {&t->gamp, sizeof(double) * nt * M * B},
+                 {&t->edot, sizeof(double) * B}, {&t->cost, sizeof(double) * B}})
+            if (rc == FH_OK) rc = tape_alloc(t, *pr.first, pr.second);
+        if (rc) {
+            release_tape_storage(t);
+            return rc;
+        }
+        t->capacity = batch;
+        *out = t;
+    }
+    (*out)->plan = p;
+    (*out)->batch = batch;
+    (*out)->swept = false;
+    return FH_OK;
+}
+
+GradBufs tape_grad_bufs(fh_tape t, int phase)
+{
+    return GradBufs{t->ckpt.p, t->tape_u.p, t->tape_h.p, (double*)t->hs.p, (double*)t->zs.p,
+                    (long long*)t->sstats.p, (int*)t->status.p, phase};
+}
+
+// Forward sweep of a tape from its kept launch (phase 1 on the tape's buffers).
+int tape_forward(fh_tape t, hipStream_t s)
+{
+    fh_plan p = t->plan;
+    const GradBufs gb = tape_grad_bufs(t, 1);
+    // the seed pointer is not read by a forward-only launch; passing the tape's
+    // seed buffer keeps the band-energy accumulation off the (optional) mask
+    return p->d.precision == FH_PREC_FP64
+               ? launch_gradient_with<double>(p, gb, t->batch, t->u0.p, t->seed.p, t->uL.p, t->lam0.p,
+                                              (double*)t->kgrad.p, (double*)t->cost.p, s)
+               : launch_gradient_with<float>(p, gb, t->batch, t->u0.p, t->seed.p, t->uL.p, t->lam0.p,
+                                             (double*)t->kgrad.p, (double*)t->cost.p, s);
+}
+
+// Status and counters of a tape's forward sweep.
+int tape_finish(fh_tape t, fh_stats* stats, bool grad)
+{
+    fh_plan p = t->plan;
+    const int batch = t->batch;
+    std::vector<int> status(batch);
+    std::vector<long long> st(3 * (size_t)batch, 0);
+    FH_HIP(hipMemcpy(status.data(), t->status.p, sizeof(int) * batch, hipMemcpyDeviceToHost));
+    if (p->adaptive) {
+        FH_HIP(hipMemcpy(st.data(), t->sstats.p, sizeof(long long) * 3 * batch, hipMemcpyDeviceToHost));
+    } else {
+        for (int b = 0; b < batch; ++b) {
+            st[3 * b] = p->d.steps;
+            st[3 * b + 2] = 4LL * p->d.steps;
+        }
+    }
+    int bad = 0;
+    for (int b = 0; b < batch; ++b) bad |= status[b];
+    const int code = (bad & 2) ? FH_EMAXSTEPS : (bad ? FH_ENONFINITE : FH_OK);
+    if (stats) {
+        stats->accepted = stats->rejected = stats->rhs_evals = 0;
+        for (int b = 0; b < batch; ++b) {
+            stats->accepted += st[3 * b];
+            stats->rejected += st[3 * b + 1];
+            stats->rhs_evals += st[3 * b + 2];
+        }
+        stats->retcode = code;
+    }
+    if (code == FH_EMAXSTEPS)
+        return fail(code, "adaptive propagation reached max_steps=" + std::to_string(p->max_steps) +
+                              " accepted steps before the fiber end");
+    if (code)
+        return fail(code, grad ? "adjoint propagation produced non-finite fields"
+                               : "propagation produced non-finite fields");
+    return FH_OK;
+}
+
 }  // namespace
 
 // Internal interface for the request server (fh_server.hip); C++ linkage.
@@ -753,13 +1029,16 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     if (!ctx || !out) return fail(FH_EINVAL, "context or out is NULL");
     int rc = check_desc(desc);
     if (rc) return rc;
+    PlanLayout L;
+    if ((rc = compute_layout(*desc, L))) return rc;
     FH_HIP(hipSetDevice(ctx->device));
     fh_plan p = new fh_plan_s;
     p->ctx = ctx;
     p->d = *desc;  // array pointers are only read during creation
     p->gamma0 = desc->gamma[0];
     p->adaptive = desc->scheme == FH_SCHEME_TSIT5;
-    rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p) : upload_tables<float>(p);
+    p->gradient_bytes = L.gradient_bytes;
+    rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p, L) : upload_tables<float>(p, L);
     p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
     p->d.band_mask = nullptr;
     if (rc) {
@@ -770,10 +1049,23 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     return FH_OK;
 }
 
+int fh_plan_check(const fh_problem_desc* desc, int32_t gradient, int64_t* bytes)
+{
+    if (!bytes) return fail(FH_EINVAL, "bytes is NULL");
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    PlanLayout L;
+    if ((rc = compute_layout(*desc, L))) return rc;
+    if (gradient && !L.grad_error.empty()) return fail(FH_EINVAL, L.grad_error);
+    *bytes = gradient ? L.gradient_bytes : L.forward_bytes;
+    return FH_OK;
+}
+
 int fh_plan_destroy(fh_plan p)
 {
     if (!p) return FH_OK;
     (void)hipSetDevice(p->ctx->device);
+    if (p->spare) release_tape_storage(p->spare);
     for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out, &p->mask,
                       &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed, &p->gr, &p->qidx, &p->hs, &p->zs, &p->sstats})
         dev_free(*b);
@@ -1148,6 +1440,123 @@ int fh_gradient(fh_plan p, int32_t batch, const double* u0, const double* seed, 
             FH_HIP(hipMemcpy(cost, p->g_cost.p, sizeof(double) * batch, hipMemcpyDeviceToHost));
     }
     return finish_call(p, batch, status, stats, true);
+}
+
+int fh_forward_tape(fh_plan p, int32_t batch, const double* u0, double* uL, fh_tape* tape, fh_stats* stats)
+{
+    if (!p || !u0 || !tape) return fail(FH_EINVAL, "plan, launch or tape pointer is NULL");
+    *tape = nullptr;
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    if (p->d.modes > 1 && !p->gamma_full_sym)
+        return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
+    const size_t n = (size_t)batch * p->d.nt * p->d.modes;
+    double norm2 = 0.0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");
+        norm2 += u0[i] * u0[i];
+    }
+    if (!(norm2 > 0)) return fail(FH_EINVAL, "input spectrum must be nonzero");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    int rc = fp64 ? ensure_grad_work<double>(p) : ensure_grad_work<float>(p);
+    if (rc) return rc;
+    fh_tape t = nullptr;
+    if ((rc = new_tape(p, batch, &t))) return rc;
+    if (!(rc = upload_field(fp64, t->u0.p, u0, n, s)) && !(rc = tape_forward(t, s))) {
+        FH_HIP(hipStreamSynchronize(s));
+        rc = tape_finish(t, stats, false);
+    }
+    if (!rc && uL) rc = download_field(fp64, uL, t->uL.p, n, s);
+    if (rc) {
+        release_tape_storage(t);
+        return rc;
+    }
+    *tape = t;
+    return FH_OK;
+}
+
+int fh_adjoint(fh_tape t, const double* seed, double* lam0, double* grad_phase, double* grad_amp, double* energy_dot,
+               double* cost, fh_stats* stats)
+{
+    if (!t || !t->plan) return fail(FH_EINVAL, "tape is NULL");
+    fh_plan p = t->plan;
+    const int batch = t->batch;
+    const size_t nt = (size_t)p->d.nt, M = (size_t)p->d.modes, n = (size_t)batch * nt * M;
+    if (!seed && !p->has_mask) return fail(FH_EINVAL, "adjoint needs a terminal seed or a plan band mask");
+    if (seed)
+        for (size_t i = 0; i < 2 * n; ++i)
+            if (!std::isfinite(seed[i])) return fail(FH_EINVAL, "adjoint terminal condition must be finite");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    int rc;
+    // a single-mode backward sweep overwrites the kept tape (each slot is
+    // pulled back once): sweeping again rebuilds it from the kept launch,
+    // which reproduces the same bits
+    if (t->swept && M == 1 && p->ntaped > 0) {
+        if ((rc = tape_forward(t, s))) return rc;
+    }
+    if (seed) {
+        if ((rc = upload_field(fp64, t->seed.p, seed, n, s))) return rc;
+    } else {
+        const double tau = p->d.objective_tau > 0 ? p->d.objective_tau : 50.0;
+        hipError_t e = fh::launch_objective(fp64, t->uL.p, (const unsigned char*)p->mask.p, (int)nt, (int)M,
+                                            p->d.objective, tau, p->d.log_cost, t->seed.p, (double*)t->cost.p, batch,
+                                            s);
+        if (e != hipSuccess) return fail(FH_EDEVICE, std::string("objective launch: ") + hipGetErrorString(e));
+    }
+    const GradBufs gb = tape_grad_bufs(t, 2);
+    rc = fp64 ? launch_gradient_with<double>(p, gb, batch, t->u0.p, t->seed.p, t->uL.p, t->lam0.p,
+                                             (double*)t->kgrad.p, (double*)t->cost.p, s)
+              : launch_gradient_with<float>(p, gb, batch, t->u0.p, t->seed.p, t->uL.p, t->lam0.p,
+                                            (double*)t->kgrad.p, (double*)t->cost.p, s);
+    if (rc) return rc;
+    t->swept = true;
+    hipError_t e = fh::launch_physical_gradients(fp64, t->lam0.p, t->u0.p, (int)(nt * M), (double*)t->gphase.p,
+                                                 (double*)t->gamp.p, (double*)t->edot.p, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("physical gradient launch: ") + hipGetErrorString(e));
+    FH_HIP(hipStreamSynchronize(s));
+    if ((rc = tape_finish(t, stats, true))) return rc;
+    if (lam0 && (rc = download_field(fp64, lam0, t->lam0.p, n, s))) return rc;
+    if (grad_phase) FH_HIP(hipMemcpy(grad_phase, t->gphase.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (grad_amp) FH_HIP(hipMemcpy(grad_amp, t->gamp.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (energy_dot) FH_HIP(hipMemcpy(energy_dot, t->edot.p, sizeof(double) * batch, hipMemcpyDeviceToHost));
+    if (cost) {
+        if (seed) {
+            for (int b = 0; b < batch; ++b) cost[b] = NAN;
+        } else {
+            FH_HIP(hipMemcpy(cost, t->cost.p, sizeof(double) * batch, hipMemcpyDeviceToHost));
+            for (int b = 0; b < batch; ++b)
+                if (!std::isfinite(cost[b]))
+                    return fail(FH_EINVAL, "objective is undefined for this output (a mode it measures has "
+                                           "zero energy)");
+        }
+    }
+    return FH_OK;
+}
+
+int fh_tape_device_bytes(fh_tape t, int64_t* bytes)
+{
+    if (!t || !bytes) return fail(FH_EINVAL, "tape or bytes is NULL");
+    *bytes = t->bytes;
+    return FH_OK;
+}
+
+int fh_tape_free(fh_tape t)
+{
+    if (!t) return FH_OK;
+    fh_plan p = t->plan;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    if (!p->spare) {
+        t->plan = nullptr;
+        p->spare = t;
+    } else {
+        release_tape_storage(t);
+    }
+    return FH_OK;
 }
 
 int fh_solve_stats(fh_plan p, int32_t batch, int64_t* stats)

@@ -7,7 +7,7 @@ run in libfiberhip's HIP kernels; this package only builds operator arrays on
 the host the way the reference's helpers do and moves fields across the ABI.
 """
 from ._lib import FiberHIPError, FiberHIPUnavailable, exported_names, lib  # noqa: F401
-from .engine import Context, DeviceArray, Plan, default_context  # noqa: F401
+from .engine import Context, DeviceArray, Plan, Tape, default_context  # noqa: F401
 from .serving import Server  # noqa: F401
 from .problem import (  # noqa: F401
     Fiber,
@@ -37,6 +37,10 @@ from .adjoint import (  # noqa: F401
     band_gradient,
     check_adjoint_gradient,
     fiber_model,
+    mmf_cost_fundamental,
+    mmf_cost_sum,
+    mmf_cost_worst_mode,
+    mmf_objective,
     raman_band_objective,
     run_adjoint_step,
     spectral_band_objective,

@@ -27,6 +27,9 @@ FH_PREC_FP64 = 0
 FH_PREC_FP32_BULK = 1
 FH_SCHEME_RK4IP_FIXED = 0
 FH_SCHEME_TSIT5 = 1
+FH_OBJ_BAND_SUM = 0
+FH_OBJ_FUNDAMENTAL = 1
+FH_OBJ_WORST_MODE = 2
 
 _NAMES = {
     FH_EINVAL: "FH_EINVAL", FH_EGRID: "FH_EGRID", FH_ENOTSYM: "FH_ENOTSYM",
@@ -69,6 +72,8 @@ class ProblemDesc(ctypes.Structure):
         ("atol", ctypes.c_double),
         ("max_steps", ctypes.c_int32),
         ("tape_segments", ctypes.c_int32),
+        ("objective", ctypes.c_int32),
+        ("objective_tau", ctypes.c_double),
     ]
 
 
@@ -143,6 +148,11 @@ _SIGNATURES = {
     "fh_plan_create": ([_VP, ctypes.POINTER(ProblemDesc), ctypes.POINTER(_VP)], ctypes.c_int),
     "fh_plan_destroy": ([_VP], ctypes.c_int),
     "fh_plan_device_bytes": ([_VP, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_plan_check": ([ctypes.POINTER(ProblemDesc), ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_forward_tape": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(_VP), ctypes.POINTER(Stats)], ctypes.c_int),
+    "fh_adjoint": ([_VP, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "fh_tape_device_bytes": ([_VP, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "fh_tape_free": ([_VP], ctypes.c_int),
     "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
     "fh_forward_saveat": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)],
                           ctypes.c_int),

@@ -103,8 +103,112 @@ def _band_objective(name, mask, log_cost):
                         log_cost=log_cost)
 
 
+def _mode_energies(u, mask):
+    u = np.asarray(u)
+    mask = np.asarray(mask, dtype=bool)
+    return np.sum(np.abs(u) ** 2, axis=0), np.sum(np.abs(u[mask, :]) ** 2, axis=0)
+
+
+def mmf_cost_sum(u, band_mask):
+    """mmf_cost_sum, mmf_cost.jl:49-62: J = sum_m E_band_m / sum_m E_m and dJ/dconj(u)."""
+    et, eb = _mode_energies(u, band_mask)
+    if not np.sum(et) > 0:
+        raise ValueError("field must have nonzero spectral energy")
+    J = float(np.sum(eb) / np.sum(et))
+    return J, u * (np.asarray(band_mask, dtype=float)[:, None] - J) / np.sum(et)
+
+
+def mmf_cost_fundamental(u, band_mask):
+    """mmf_cost_fundamental, mmf_cost.jl:81-98: the band fraction of mode 1 only."""
+    et, eb = _mode_energies(u, band_mask)
+    if not et[0] > 0:
+        raise ValueError("fundamental mode has zero energy")
+    J = float(eb[0] / et[0])
+    seed = np.zeros_like(np.asarray(u, dtype=np.complex128))
+    seed[:, 0] = u[:, 0] * (np.asarray(band_mask, dtype=float) - J) / et[0]
+    return J, seed
+
+
+def mmf_cost_worst_mode(u, band_mask, tau: float = 50.0):
+    """mmf_cost_worst_mode, mmf_cost.jl:124-166: log-sum-exp proxy of the worst
+    mode's band fraction over the modes with energy, and its softmax-weighted seed."""
+    if not (np.isfinite(tau) and tau > 0):
+        raise ValueError("tau must be positive and finite")
+    et, eb = _mode_energies(u, band_mask)
+    active = np.flatnonzero(et > 0)
+    if active.size == 0:
+        raise ValueError("worst-mode cost requires a nonzero-energy mode")
+    r = np.where(et > 0, eb / np.where(et > 0, et, 1.0), 0.0)
+    rmax = float(np.max(r[active]))
+    shifted = tau * (r[active] - rmax)
+    denom = float(np.sum(np.exp(shifted)))
+    J = rmax + (np.log(denom) - np.log(active.size)) / tau
+    seed = np.zeros_like(np.asarray(u, dtype=np.complex128))
+    m = np.asarray(band_mask, dtype=float)
+    for w, k in zip(np.exp(shifted) / denom, active):
+        seed[:, k] = w * u[:, k] * (m - r[k]) / et[k]
+    return float(J), seed
+
+
+def mmf_objective(problem: FiberFieldProblem, kind: str = "sum", band=None, tau: float = 50.0) -> ObjectiveMap:
+    """Multimode Raman-band objective (mmf_cost.jl): kind "sum", "fundamental"
+    or "worst_mode", over the problem's Raman band or `band` (THz)."""
+    mask = problem.band_mask if band is None else frequency_band_mask(problem, band)
+    if mask is None:
+        raise ValueError("problem has no Raman band mask; pass band")
+    mask = np.asarray(mask, dtype=bool)
+    fns = {"sum": lambda u: mmf_cost_sum(u, mask), "fundamental": lambda u: mmf_cost_fundamental(u, mask),
+           "worst_mode": lambda u: mmf_cost_worst_mode(u, mask, tau)}
+    if kind not in fns:
+        raise ValueError(f"kind must be one of {sorted(fns)}")
+    f = fns[kind]
+    obj = ObjectiveMap(f"mmf_{kind}", cost=lambda u: f(u)[0], terminal_adjoint=lambda u: f(u)[1], band_mask=mask)
+    obj.device_objective = "band_sum" if kind == "sum" else kind
+    obj.tau = tau
+    return obj
+
+
+def _field_matrix(value, nt: int, m: int, label: str):
+    """_phase_matrix / _field_matrix, physics_models.jl:690-730: a scalar, an
+    (nt,) vector repeated over modes or an (nt, m) matrix, and its shape tag."""
+    a = np.asarray(value, dtype=np.float64)
+    if a.ndim == 0:
+        out, shape = np.full((nt, m), float(a)), "scalar"
+    elif a.ndim == 1:
+        if a.shape != (nt,):
+            raise ValueError(f"{label} vector length {a.shape[0]} does not match Nt={nt}")
+        out, shape = np.repeat(a[:, None], m, axis=1), "vector"
+    elif a.ndim == 2:
+        if a.shape != (nt, m):
+            raise ValueError(f"{label} matrix size {a.shape} does not match ({nt}, {m})")
+        out, shape = a.copy(), "matrix"
+    else:
+        raise ValueError(f"{label} must be a real scalar, vector, or matrix")
+    if not np.all(np.isfinite(out)):
+        raise ValueError(f"{label} values must be finite")
+    return out, shape
+
+
+def _gradient_for_shape(g, shape: str):
+    """_physical_gradient_for_shape, physics_models.jl:732-737."""
+    if shape == "scalar":
+        return np.array([np.sum(g)])
+    if shape == "vector":
+        return np.sum(g, axis=1)
+    return g
+
+
 class AdjointModel:
-    """GPU spectral-shaper model: launch alpha*A*cis(phi)*uw0 (here alpha = A = 1).
+    """GPU spectral-shaper model (spectral_shaper_model, physics_models.jl:1022-1081).
+
+    The launch is alpha A cis(phi) uw0 with alpha = sqrt(energy) for a decoded
+    control that is a phase array or a dict with "phase", "amplitude" and
+    "energy" (_single_mode_fields, physics_models.jl:751-773).  forward() keeps
+    the solve on the device as a tape (_cached_forward!, :799-819); the
+    physical gradient sweeps backward over that tape, so a forward plus a
+    gradient costs one forward sweep, and returns the phase, amplitude and
+    energy gradients of physics_models.jl:1042-1047 for the keys the control
+    has (the phase gradient alone for an array control).
 
     tape_segments: checkpoint segments whose stage tape the forward sweep keeps
     (fewer recomputations, more device memory; gradients are bit-identical).
@@ -128,7 +232,10 @@ class AdjointModel:
         self.tape_segments = tape_segments
         self.plan = Plan.from_problem(problem, steps=steps, precision=precision, band_mask=mask,
                                       checkpoint_every=checkpoint_every, tape_segments=tape_segments,
-                                      log_cost=bool(objective.log_cost) if objective is not None else False)
+                                      log_cost=bool(objective.log_cost) if objective is not None else False,
+                                      objective=getattr(objective, "device_objective", "band_sum"),
+                                      objective_tau=getattr(objective, "tau", 50.0))
+        self._cache = None  # (signature, tape, fields)
 
     @classmethod
     def default_tape_segments(cls, problem: FiberFieldProblem, steps: int, precision: str,
@@ -148,17 +255,69 @@ class AdjointModel:
         nseg = -(-int(steps) // every)
         return int(max(0, min(nseg, cls.TAPE_BUDGET_BYTES // per_segment - 1)))
 
-    def shaped_input(self, phase) -> np.ndarray:
-        """_shaped_input, physics_models.jl:791-797 (alpha = A = 1)."""
-        return self.problem.uω0 * np.exp(1j * np.asarray(phase))[:, None]
+    def fields(self, decoded) -> dict:
+        """Phase, amplitude and energy of a decoded control (_single_mode_fields)."""
+        nt, m = self.problem.nt, self.problem.modes
+        get = (lambda k, d: decoded.get(k, d)) if isinstance(decoded, dict) else (
+            lambda k, d: decoded if k == "phase" else d)
+        phase, phase_shape = _field_matrix(get("phase", np.zeros((nt, m))), nt, m, "phase")
+        amplitude, amplitude_shape = _field_matrix(get("amplitude", 1.0), nt, m, "amplitude")
+        if not np.all(amplitude > 0):
+            raise ValueError("fiber field amplitude control must be strictly positive")
+        energy = float(get("energy", 1.0))
+        if not (np.isfinite(energy) and energy > 0):
+            raise ValueError("fiber field energy control must be positive and finite")
+        return dict(phase=phase, phase_shape=phase_shape, amplitude=amplitude, amplitude_shape=amplitude_shape,
+                    energy=energy, alpha=np.sqrt(energy))
 
-    def forward(self, phase) -> np.ndarray:
-        out, _ = self.plan.forward(self.shaped_input(phase))
+    def shaped_input(self, decoded) -> np.ndarray:
+        """_shaped_input, physics_models.jl:791-797: alpha A cis(phi) uw0."""
+        f = decoded if isinstance(decoded, dict) and "alpha" in decoded else self.fields(decoded)
+        shaped = f["alpha"] * f["amplitude"] * np.exp(1j * f["phase"]) * self.problem.uω0
+        if not (np.all(np.isfinite(shaped)) and np.linalg.norm(shaped) > 0):
+            raise ValueError("decoded controls must produce a finite, nonzero launch field")
+        return shaped
+
+    def _cached_forward(self, decoded):
+        f = self.fields(decoded)
+        sig = (f["phase"].tobytes(), f["amplitude"].tobytes(), f["energy"])
+        if self._cache is None or self._cache[0] != sig:
+            if self._cache is not None:
+                self._cache[1].free()
+                self._cache = None
+            self._cache = (sig, self.plan.forward_tape(self.shaped_input(f)), f)
+        return self._cache[1], self._cache[2]
+
+    def forward(self, decoded) -> np.ndarray:
+        return self._cached_forward(decoded)[0].output
+
+    def physical_gradient(self, decoded, terminal_seed):
+        tape, f = self._cached_forward(decoded)
+        res = tape.adjoint(terminal_seed)
+        phase_g = res["grad_phase"]
+        if not isinstance(decoded, dict):
+            return _gradient_for_shape(phase_g, f["phase_shape"])
+        out = {}
+        if "phase" in decoded:
+            out["phase"] = _gradient_for_shape(phase_g, f["phase_shape"])
+        if "amplitude" in decoded:
+            out["amplitude"] = _gradient_for_shape(res["grad_amplitude"] / f["amplitude"], f["amplitude_shape"])
+        if "energy" in decoded:
+            out["energy"] = np.array([float(res["energy_dot"]) / f["energy"]])
         return out
 
-    def physical_gradient(self, phase, terminal_seed) -> np.ndarray:
-        res = self.plan.gradient(self.shaped_input(phase), seed=terminal_seed)
-        return res["grad"]
+    def device_objective_gradient(self, decoded) -> tuple[float, np.ndarray]:
+        """Cost and phase gradient of the plan's objective computed on the
+        device from the kept output (no host seed)."""
+        tape, f = self._cached_forward(decoded)
+        res = tape.adjoint(None)
+        return float(res["cost"]), _gradient_for_shape(res["grad_phase"], f["phase_shape"])
+
+    def close(self) -> None:
+        if self._cache is not None:
+            self._cache[1].free()
+            self._cache = None
+        self.plan.close()
 
 
 def _gamma_symmetric(g) -> bool:

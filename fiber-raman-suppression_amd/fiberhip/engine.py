@@ -11,6 +11,9 @@ from ._lib import FH_PREC_FP32_BULK, FH_PREC_FP64, FH_SCHEME_RK4IP_FIXED, FH_SCH
 
 PRECISIONS = {"fp64": FH_PREC_FP64, "fp32": FH_PREC_FP32_BULK, "fp32_bulk": FH_PREC_FP32_BULK}
 SCHEMES = {"rk4ip": FH_SCHEME_RK4IP_FIXED, "tsit5": FH_SCHEME_TSIT5}
+# terminal objectives of gradients without a caller seed (mmf_cost.jl)
+OBJECTIVES = {"band_sum": _lib.FH_OBJ_BAND_SUM, "fundamental": _lib.FH_OBJ_FUNDAMENTAL,
+              "worst_mode": _lib.FH_OBJ_WORST_MODE}
 
 
 def _dp(a: np.ndarray):
@@ -106,8 +109,37 @@ class Plan:
     def __init__(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: Optional[int],
                  precision: str = "fp64", max_batch: int = 1, context: Optional[Context] = None,
                  band_mask=None, checkpoint_every: int = 50, log_cost: bool = False, scheme: Optional[str] = None,
-                 rtol: float = 1e-8, atol: float = 1e-6, max_steps: int = 10000, tape_segments: int = 0):
+                 rtol: float = 1e-8, atol: float = 1e-6, max_steps: int = 10000, tape_segments: int = 0,
+                 objective: str = "band_sum", objective_tau: float = 50.0):
+        desc = self._describe(dispersion, gamma, raman, shock, one_m_fr, length_m, steps, precision, max_batch,
+                              band_mask, checkpoint_every, log_cost, scheme, rtol, atol, max_steps, tape_segments,
+                              objective, objective_tau)
         self.context = context or default_context()
+        h = ctypes.c_void_p()
+        check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def check(cls, *args, gradient: bool = False, **kwargs) -> int:
+        """Device bytes a plan with these arguments holds (fh_plan_check; host
+        only, no device touched), including its adjoint buffers with
+        `gradient`; raises FiberHIPError as fh_plan_create would."""
+        kwargs.pop("context", None)
+        self = cls.__new__(cls)
+        desc = self._describe(*args, **kwargs)
+        n = ctypes.c_int64()
+        check(lib().fh_plan_check(ctypes.byref(desc), int(bool(gradient)), ctypes.byref(n)))
+        return n.value
+
+    def _describe(self, dispersion, gamma, raman, shock, one_m_fr: float, length_m: float, steps: Optional[int],
+                  precision: str = "fp64", max_batch: int = 1, band_mask=None, checkpoint_every: int = 50,
+                  log_cost: bool = False, scheme: Optional[str] = None, rtol: float = 1e-8, atol: float = 1e-6,
+                  max_steps: int = 10000, tape_segments: int = 0, objective: str = "band_sum",
+                  objective_tau: float = 50.0) -> "_lib.ProblemDesc":
+        """Resolve the arguments into an fh_problem_desc (arrays kept alive on self)."""
+        if objective not in OBJECTIVES:
+            raise ValueError(f"objective must be one of {sorted(OBJECTIVES)}")
+        self.objective = objective
         D = np.asarray(dispersion, dtype=np.float64)
         if D.ndim == 1:
             D = D[:, None]
@@ -137,16 +169,16 @@ class Plan:
             shock=_dp(self._sh), one_m_fr=float(one_m_fr), checkpoint_every=int(checkpoint_every),
             log_cost=int(bool(log_cost)),
             band_mask=None if self._mask is None else self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-            rtol=self.rtol, atol=self.atol, max_steps=self.max_steps, tape_segments=int(tape_segments))
-        h = ctypes.c_void_p()
-        check(lib().fh_plan_create(self.context.handle, ctypes.byref(desc), ctypes.byref(h)))
-        self.handle = h
+            rtol=self.rtol, atol=self.atol, max_steps=self.max_steps, tape_segments=int(tape_segments),
+            objective=OBJECTIVES[objective], objective_tau=float(objective_tau))
+        return desc
 
     @classmethod
     def from_problem(cls, problem, steps: Optional[int] = None, precision: str = "fp64", max_batch: int = 1,
                      context: Optional[Context] = None, band_mask=None, checkpoint_every: int = 50,
                      log_cost: bool = False, scheme: Optional[str] = None, rtol: Optional[float] = None,
-                     atol: Optional[float] = None, max_steps: int = 10000, tape_segments: int = 0) -> "Plan":
+                     atol: Optional[float] = None, max_steps: int = 10000, tape_segments: int = 0,
+                     objective: str = "band_sum", objective_tau: float = 50.0) -> "Plan":
         """Plan for a FiberFieldProblem.  With `steps` the fixed-step RK4IP of
         _scalar_rk4_output; without, the adaptive Tsit5 of solve_disp_mmf with the
         fiber's reltol/abstol (defaults 1e-8 / 1e-6, simulate_disp_mmf.jl:186-190)."""
@@ -158,7 +190,8 @@ class Plan:
         return cls(f["Dω"], f["γ"], f["hRω"], self_steepening(problem.sim), f["one_m_fR"], f["L"], steps,
                    precision=precision, max_batch=max_batch, context=context, band_mask=band_mask,
                    checkpoint_every=checkpoint_every, log_cost=log_cost, scheme=scheme, rtol=rtol, atol=atol,
-                   max_steps=max_steps, tape_segments=tape_segments)
+                   max_steps=max_steps, tape_segments=tape_segments, objective=objective,
+                   objective_tau=objective_tau)
 
     def solve_stats(self, batch: int) -> np.ndarray:
         """(batch, 3) accepted, rejected, rhs evaluations of the last call (fh_solve_stats)."""
@@ -274,6 +307,26 @@ class Plan:
             out = {k: (v[0] if isinstance(v, np.ndarray) else v) for k, v in out.items()}
         return out
 
+    def forward_tape(self, u0) -> "Tape":
+        """Forward solve that keeps its tape on the device (fh_forward_tape);
+        Tape.adjoint() then sweeps backward without a second forward solve."""
+        u = np.asarray(u0, dtype=np.complex128)
+        single = u.ndim == 2
+        if single:
+            u = u[None]
+        if u.ndim != 3 or u.shape[1:] != (self.nt, self.modes):
+            raise ValueError(f"launch has shape {u.shape}, expected (batch, {self.nt}, {self.modes})")
+        batch = u.shape[0]
+        buf = np.ascontiguousarray(np.transpose(u, (0, 2, 1)))
+        uL = np.empty_like(buf)
+        h = ctypes.c_void_p()
+        st = _lib.Stats()
+        check(lib().fh_forward_tape(self.handle, batch, _dp(buf.view(np.float64)), _dp(uL.view(np.float64)),
+                                    ctypes.byref(h), ctypes.byref(st)))
+        out = np.transpose(uL, (0, 2, 1))
+        stats = dict(accepted=st.accepted, rejected=st.rejected, rhs_evaluations=st.rhs_evals, retcode=st.retcode)
+        return Tape(self, h, batch, single, out[0] if single else out, stats)
+
     def time_gradient_device(self, batch: int, d_u0: int, d_uL: int, d_lam0: int, d_grad: int, d_cost: int,
                              launches: int) -> np.ndarray:
         ms = (ctypes.c_float * launches)()
@@ -325,5 +378,64 @@ class Plan:
     def __del__(self):  # pragma: no cover
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Tape:
+    """A forward solve kept on the device (fh_forward_tape) for later backward
+    sweeps (fh_adjoint), the cached forward of the reference's adjoint
+    contract (_cached_forward!, physics_models.jl:799-819)."""
+
+    def __init__(self, plan: Plan, handle, batch: int, single: bool, output: np.ndarray, stats: dict):
+        self.plan, self.handle, self.batch, self.single = plan, handle, batch, single
+        self.output, self.stats = output, stats
+
+    def adjoint(self, seed=None) -> dict:
+        """Backward sweep with Wirtinger terminal seeds dJ/dconj(uL) (shape of
+        the launch), or None for the plan's objective computed on the device.
+        Returns lam0, grad_phase = 2 Re(conj(lam0) i u0), grad_amplitude =
+        2 Re(conj(lam0) u0) (per mode), energy_dot = Re sum conj(lam0) u0 and
+        cost (NaN with a seed)."""
+        if not self.handle:
+            raise ValueError("tape was freed")
+        p, B = self.plan, self.batch
+        sd = None
+        if seed is not None:
+            sv = np.asarray(seed, dtype=np.complex128)
+            sv = sv[None] if sv.ndim == 2 else sv
+            if sv.shape != (B, p.nt, p.modes):
+                raise ValueError("seed must match the launch shape")
+            sd = np.ascontiguousarray(np.transpose(sv, (0, 2, 1)))
+        lam0 = np.empty((B, p.modes, p.nt), dtype=np.complex128)
+        gph = np.empty((B, p.modes, p.nt))
+        gam = np.empty((B, p.modes, p.nt))
+        edot = np.empty(B)
+        cost = np.empty(B)
+        st = _lib.Stats()
+        check(lib().fh_adjoint(self.handle, None if sd is None else _dp(sd.view(np.float64)),
+                               _dp(lam0.view(np.float64)), _dp(gph), _dp(gam), _dp(edot), _dp(cost),
+                               ctypes.byref(st)))
+        out = dict(lam0=np.transpose(lam0, (0, 2, 1)), grad_phase=np.transpose(gph, (0, 2, 1)),
+                   grad_amplitude=np.transpose(gam, (0, 2, 1)), energy_dot=edot, cost=cost)
+        if self.single:
+            out = {k: v[0] for k, v in out.items()}
+        return out
+
+    @property
+    def device_bytes(self) -> int:
+        n = ctypes.c_int64()
+        check(lib().fh_tape_device_bytes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def free(self) -> None:
+        if getattr(self, "handle", None):
+            lib().fh_tape_free(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if self.plan.handle:
+                self.free()
         except Exception:
             pass

@@ -70,6 +70,17 @@ enum { FH_PREC_FP64 = 0, FH_PREC_FP32_BULK = 1 };
  * counts of the library (FH_EUNSUPPORTED otherwise). */
 enum { FH_SCHEME_RK4IP_FIXED = 0, FH_SCHEME_TSIT5 = 1 };
 
+/* Terminal objective of a gradient without a caller seed (all use the plan's
+ * band mask; each returns J and its Wirtinger seed dJ/dconj(uL)):
+ *   FH_OBJ_BAND_SUM     sum_m E_band_m / sum_m E_m  (_field_spectral_band_cost,
+ *                       physics_helpers.jl:222-232; mmf_cost_sum, mmf_cost.jl:49-62)
+ *   FH_OBJ_FUNDAMENTAL  E_band_1 / E_1 of mode 1 (mmf_cost_fundamental, mmf_cost.jl:81-98)
+ *   FH_OBJ_WORST_MODE   log-sum-exp proxy of max_m E_band_m / E_m with
+ *                       temperature objective_tau (mmf_cost_worst_mode,
+ *                       mmf_cost.jl:124-166)
+ * For one mode the three coincide. */
+enum { FH_OBJ_BAND_SUM = 0, FH_OBJ_FUNDAMENTAL = 1, FH_OBJ_WORST_MODE = 2 };
+
 typedef struct fh_context_s *fh_context;
 typedef struct fh_plan_s *fh_plan;
 
@@ -103,7 +114,14 @@ typedef struct {
                                * stage's tape (single mode: nt complex + nt real;
                                * multimode: (M + ceil(Q/2)) * nt complex) per solve of
                                * max_batch.  Results are bit-identical for every K. */
+    int32_t objective;        /* FH_OBJ_* of gradients without a caller seed     */
+    double objective_tau;     /* FH_OBJ_WORST_MODE temperature (0 = 50)          */
 } fh_problem_desc;
+
+/* Per-solve device regions of a gradient (checkpoints, each stage tape) are
+ * addressed with 32-bit offsets, so each must stay below 4 GiB per solve;
+ * plans whose gradient would exceed that are rejected with FH_EINVAL at the
+ * first gradient call (or by fh_plan_check). */
 
 /* Counters of one call, summed over its batch (per-solve values: fh_solve_stats). */
 typedef struct {
@@ -126,6 +144,14 @@ int fh_plan_destroy(fh_plan plan);
 
 /* Device bytes the plan holds (operators plus scratch for max_batch). */
 int fh_plan_device_bytes(fh_plan plan, int64_t *bytes);
+
+/* Validates a descriptor on the host alone (no device is touched) and returns
+ * the device bytes a plan of it holds: *bytes for forward use, and with
+ * `gradient` != 0 including the adjoint buffers of its first gradient call
+ * (checkpoints, stage tape, adjoint fields).  Returns the error fh_plan_create
+ * (or, with `gradient`, the first fh_gradient) would return.  The server uses
+ * it for admission by memory footprint. */
+int fh_plan_check(const fh_problem_desc *desc, int32_t gradient, int64_t *bytes);
 
 /* Forward solve of `batch` independent launches sharing the plan's fiber.
  * u0: batch*modes*nt complex128 launch spectra (interaction picture at z=0,
@@ -181,6 +207,49 @@ int fh_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, const void
 /* Event-timed back-to-back gradient launches (see fh_time_forward_device). */
 int fh_time_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *d_lam0,
                             double *d_grad, double *d_cost, int32_t launches, float *ms_out);
+
+/* ---------------------------------------------------------- kept forward
+ *
+ * The reference's adjoint contract runs the forward solve once and the
+ * adjoint later on the cached solution (_cached_forward! and the
+ * spectral_shaper_model physical_gradient, physics_models.jl:799-819,
+ * 1032-1047; solve_adjoint_disp_mmf on the forward ODESolution,
+ * sensitivity_disp_mmf.jl:309-332).  A tape is that cached forward on the
+ * device: the shaped launch, the output, the checkpoints, the kept stage tape
+ * of tape_segments segments and (adaptive plans) the accepted steps.
+ * fh_forward_tape + fh_adjoint with a seed give the bits of fh_gradient with
+ * the same seed, with one forward sweep in total.  A tape belongs to its plan
+ * (free it before the plan) and its calls run on the plan's stream. */
+
+typedef struct fh_tape_s *fh_tape;
+
+/* Forward solve of `batch` launches (host complex128, as fh_forward) that
+ * keeps the tape; uL (may be NULL) receives the lab-frame outputs. */
+int fh_forward_tape(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_tape *tape, fh_stats *stats);
+
+/* Backward sweep over a tape.  seed: host complex128 [batch][modes][nt]
+ * Wirtinger terminal seeds dJ/dconj(uL) (terminal_adjoint), or NULL for the
+ * plan's objective (FH_OBJ_*, computed on the device from the kept output;
+ * needs a band mask).  Outputs (each may be NULL), FP64 accumulation:
+ *   lam0         [batch][modes][nt] complex128  dJ/dconj(u0)
+ *   grad_phase   [batch][modes][nt]  2 Re(conj(lam0) i u0)  (phase gradient)
+ *   grad_amp     [batch][modes][nt]  2 Re(conj(lam0) u0)    (amplitude gradient
+ *                                    times the amplitude A of the launch)
+ *   energy_dot   [batch]             Re sum conj(lam0) u0   (energy gradient
+ *                                    times the launch energy)
+ *   cost         [batch]             J of the plan's objective (NaN with a seed)
+ * physics_models.jl:1042-1047 forms the three physical gradients from lam0
+ * this way.  A tape may be swept more than once (e.g. with different seeds);
+ * single-mode sweeps consume the kept tape, so a repeated single-mode sweep
+ * first re-runs the forward sweep from the kept launch on the device. */
+int fh_adjoint(fh_tape tape, const double *seed, double *lam0, double *grad_phase, double *grad_amp,
+               double *energy_dot, double *cost, fh_stats *stats);
+
+/* Device bytes the tape holds. */
+int fh_tape_device_bytes(fh_tape tape, int64_t *bytes);
+
+/* Releases a tape (its memory may be kept by the plan for the next tape). */
+int fh_tape_free(fh_tape tape);
 
 /* FP64 per-mode spectral energy E_m = sum_w |u(w,m)|^2 of `batch` device
  * fields in the plan's precision, written to host energies[batch*modes]

@@ -248,3 +248,36 @@ def rk4_adjoint(x0, lam_out, length, steps, D, gamma, hR, shock_, one_m_fR):
 def phase_gradient(lam0: np.ndarray, u0_shaped: np.ndarray) -> np.ndarray:
     """g_phi = 2 Re(conj(lam0) i u0), physics_models.jl:1042 (summed over modes)."""
     return np.sum(2 * np.real(np.conj(lam0) * 1j * u0_shaped), axis=1)
+
+
+# ------------------------------------------------------ multimode costs
+
+
+def mmf_costs(u: np.ndarray, mask: np.ndarray, kind: str, tau: float = 50.0):
+    """mmf_cost_sum / mmf_cost_fundamental / mmf_cost_worst_mode (mmf_cost.jl:
+    49-62, 81-98, 124-166) -> (J, dJ/dconj(u)), written mode by mode."""
+    nt, m = u.shape
+    e_tot = [float(np.vdot(u[:, k], u[:, k]).real) for k in range(m)]
+    e_band = [float(np.vdot(u[mask, k], u[mask, k]).real) for k in range(m)]
+    ind = mask.astype(float)
+    seed = np.zeros((nt, m), dtype=complex)
+    if kind == "sum":
+        J = sum(e_band) / sum(e_tot)
+        for k in range(m):
+            seed[:, k] = u[:, k] * (ind - J) / sum(e_tot)
+        return J, seed
+    if kind == "fundamental":
+        J = e_band[0] / e_tot[0]
+        seed[:, 0] = u[:, 0] * (ind - J) / e_tot[0]
+        return J, seed
+    if kind == "worst_mode":
+        act = [k for k in range(m) if e_tot[k] > 0]
+        r = {k: e_band[k] / e_tot[k] for k in act}
+        rmax = max(r.values())
+        z = sum(math.exp(tau * (r[k] - rmax)) for k in act)
+        J = rmax + (math.log(z) - math.log(len(act))) / tau
+        for k in act:
+            w = math.exp(tau * (r[k] - rmax)) / z
+            seed[:, k] = w * u[:, k] * (ind - r[k]) / e_tot[k]
+        return J, seed
+    raise ValueError(kind)

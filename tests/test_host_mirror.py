@@ -107,8 +107,10 @@ def test_library_loads_and_exports_header_symbols():
 
 def test_struct_layout_matches_header():
     """ProblemDesc: 6 int32, length_m, 4 pointers, one_m_fr, 2 int32, band_mask
-    pointer, rtol, atol, max_steps, tape_segments."""
-    assert ctypes.sizeof(_lib.ProblemDesc) == 6 * 4 + 8 + 4 * 8 + 8 + 2 * 4 + 8 + 8 + 8 + 2 * 4
+    pointer, rtol, atol, max_steps, tape_segments, objective (+4 padding),
+    objective_tau."""
+    assert ctypes.sizeof(_lib.ProblemDesc) == 6 * 4 + 8 + 4 * 8 + 8 + 2 * 4 + 8 + 8 + 8 + 2 * 4 + 8 + 8
+    assert _lib.ProblemDesc.objective.offset == 112 and _lib.ProblemDesc.objective_tau.offset == 120
     assert _lib.ProblemDesc.band_mask.offset == 80
     assert _lib.ProblemDesc.rtol.offset == 88 and _lib.ProblemDesc.max_steps.offset == 104
     assert _lib.ProblemDesc.tape_segments.offset == 108
@@ -211,3 +213,121 @@ def test_adjoint_model_default_tape_budget():
     for prec, esz in (("fp64", 8), ("fp32", 4)):
         k = AdjointModel.default_tape_segments(p, 20000, prec, 50)
         assert (k + 1) * 4 * 50 * 16384 * 3 * esz <= AdjointModel.TAPE_BUDGET_BYTES
+
+
+def _check_args(nt, modes=1, precision="fp64", steps=2000, scheme=None, checkpoint_every=50, max_steps=10000,
+                tape_segments=0, band=True, max_batch=1):
+    from oracle import np_ref as R
+
+    sim = R.sim_params(1550e-9, modes, nt, 1.2 * nt / (2 * 193.41449), 3)
+    rf = R.raman_fields(sim, 0.18)
+    D = np.repeat(R.dispersion_user(sim, [-2.17e-26, 1.2e-40]), modes, axis=1)
+    g = np.full((modes,) * 4, 1e-3)
+    mask = np.zeros(nt, dtype=bool)
+    mask[: nt // 8] = True
+    return (D, g, rf["hR"], R.shock(sim), rf["one_m_fR"], 1.0, steps), dict(
+        precision=precision, scheme=scheme, checkpoint_every=checkpoint_every, max_steps=max_steps,
+        tape_segments=tape_segments, band_mask=mask if band else None, max_batch=max_batch)
+
+
+def test_plan_check_rejects_gradients_beyond_32bit_regions():
+    """fh_plan_check (host only): a forward plan is fine, but a gradient whose
+    checkpoints or stage tape of one solve exceed 4 GiB is rejected with
+    FH_EINVAL instead of addressing past the 32-bit buffer range."""
+    from fiberhip.engine import Plan
+
+    # FP64, 16384 samples, 40000 steps checkpointed every step: 40000 * 256 KiB = 10.5 GB per solve
+    args, kw = _check_args(16384, steps=40000, checkpoint_every=1)
+    assert Plan.check(*args, **kw) > 0
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL.*checkpoints of one solve exceed 4 GiB"):
+        Plan.check(*args, gradient=True, **kw)
+    # adaptive FP32 M = 10 at 32768 samples, max_steps 10000, every 5: 2000 checkpoints of 2.6 MB
+    args, kw = _check_args(32768, modes=10, precision="fp32", scheme="tsit5", steps=None, checkpoint_every=5,
+                           max_steps=10000)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL.*checkpoints"):
+        Plan.check(*args, gradient=True, **kw)
+    # stage tape: each 2000-step region of the FP64 16384 tape holds 2.1 GB of u, seven regions pass 4 GiB
+    args, kw = _check_args(16384, steps=20000, checkpoint_every=2000, tape_segments=6)
+    with pytest.raises(fiberhip.FiberHIPError, match="stage tape of one solve exceeds 4 GiB"):
+        Plan.check(*args, gradient=True, **kw)
+    # the SMF-28 2 m gradient plan fits and its adjoint part is the layout's
+    args, kw = _check_args(16384, steps=2000, checkpoint_every=50, tape_segments=40, max_batch=4)
+    fwd, grad = Plan.check(*args, **kw), Plan.check(*args, gradient=True, **kw)
+    ckpt = 40 * 16384 * 16
+    tape = 40 * 4 * 50 * 16384 * (16 + 8)
+    assert grad - fwd == 4 * (ckpt + tape + 2 * 16384 * 16 + 16384 * 8 + 8)
+
+
+def test_plan_check_validates_like_plan_create():
+    """The descriptor checks of fh_plan_create run on the host: unknown
+    objectives, a negative temperature and a zero step count are rejected."""
+    from fiberhip.engine import Plan
+
+    args, kw = _check_args(1024, steps=100)
+    holder = Plan.__new__(Plan)
+    desc = holder._describe(*args, **kw)
+    n = ctypes.c_int64()
+    assert fiberhip.lib().fh_plan_check(ctypes.byref(desc), 0, ctypes.byref(n)) == _lib.FH_OK and n.value > 0
+    desc.objective = 7
+    assert fiberhip.lib().fh_plan_check(ctypes.byref(desc), 0, ctypes.byref(n)) == _lib.FH_EINVAL
+    assert b"unknown objective" in fiberhip.lib().fh_last_error()
+    desc.objective = _lib.FH_OBJ_WORST_MODE
+    desc.objective_tau = -1.0
+    assert fiberhip.lib().fh_plan_check(ctypes.byref(desc), 0, ctypes.byref(n)) == _lib.FH_EINVAL
+    with pytest.raises(ValueError):
+        Plan.check(*args, **kw, objective="nope")
+    args, kw = _check_args(1024, steps=0)
+    with pytest.raises(fiberhip.FiberHIPError, match="FH_EINVAL"):
+        Plan.check(*args, **kw)
+
+
+def test_adjoint_model_default_tape_checkpoint_every_zero():
+    """checkpoint_every = 0 follows the plan's rule (50, capped at the step count)."""
+    from conftest import smf_problem
+    from fiberhip.adjoint import AdjointModel
+
+    p = smf_problem(nt=16384, window_ps=47.061, length_m=2.0, power_w=0.2)
+    for steps in (2000, 20000, 30):
+        for prec in ("fp64", "fp32"):
+            assert AdjointModel.default_tape_segments(p, steps, prec, 0) == \
+                AdjointModel.default_tape_segments(p, steps, prec, 50)
+
+
+def test_mmf_costs_match_oracle():
+    """Host objectives (fiberhip.adjoint.mmf_cost_*) against the oracle's restatement."""
+    from oracle import np_ref as R
+
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=(128, 6)) + 1j * rng.normal(size=(128, 6))
+    mask = np.zeros(128, dtype=bool)
+    mask[90:110] = True
+    for kind, fn in (("sum", fiberhip.mmf_cost_sum), ("fundamental", fiberhip.mmf_cost_fundamental),
+                     ("worst_mode", fiberhip.mmf_cost_worst_mode)):
+        J, s = fn(u, mask)
+        Jr, sr = R.mmf_costs(u, mask, kind)
+        assert abs(J - Jr) <= 1e-15 * abs(Jr)
+        assert np.allclose(s, sr, rtol=1e-13, atol=1e-18)
+
+
+def test_adjoint_model_control_fields():
+    """_single_mode_fields shapes: vector phase repeats over modes, scalars fill,
+    non-positive amplitude and energy are rejected."""
+    from conftest import smf_problem
+    from fiberhip.adjoint import AdjointModel, _field_matrix, _gradient_for_shape
+
+    a, shape = _field_matrix(np.arange(4.0), 4, 3, "phase")
+    assert shape == "vector" and a.shape == (4, 3) and np.all(a[:, 2] == np.arange(4.0))
+    assert _field_matrix(2.0, 4, 3, "amplitude")[1] == "scalar"
+    g = np.arange(12.0).reshape(4, 3)
+    assert np.all(_gradient_for_shape(g, "vector") == g.sum(axis=1))
+    assert _gradient_for_shape(g, "scalar")[0] == g.sum()
+    with pytest.raises(ValueError, match="vector length"):
+        _field_matrix(np.zeros(5), 4, 3, "phase")
+    m = AdjointModel.__new__(AdjointModel)
+    m.problem = smf_problem(nt=256, window_ps=3.0, length_m=0.01)
+    with pytest.raises(ValueError, match="strictly positive"):
+        m.fields({"amplitude": 0.0})
+    with pytest.raises(ValueError, match="energy control"):
+        m.fields({"energy": -1.0})
+    f = m.fields({"phase": np.zeros(256), "energy": 4.0})
+    assert f["alpha"] == 2.0 and f["phase_shape"] == "vector"

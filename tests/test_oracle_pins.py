@@ -583,3 +583,46 @@ def test_multimode_cw_closed_form_catches_dropped_exchange_terms():
                 bad[i, j, i, j] = bad[i, j, j, i] = 0.0
     out = oracle.rk4_forward(oracle.Fiber(D, bad, hR, S, omf), L, 400, u0)
     assert _rel(out, expected) > 0.1
+
+
+# ------------------------------------------------------ multimode costs
+
+
+@pytest.mark.parametrize("kind", ["sum", "fundamental", "worst_mode"])
+def test_mmf_cost_seed_is_wirtinger_derivative(kind):
+    """The seeds of mmf_cost.jl's three costs (restated in np_ref.mmf_costs)
+    are dJ/dconj(u): for any perturbation du, dJ = 2 Re sum conj(seed) du
+    (central differences, relative 1e-7)."""
+    rng = np.random.default_rng(11)
+    nt, m = 64, 3
+    u = (rng.normal(size=(nt, m)) + 1j * rng.normal(size=(nt, m))) * np.array([1.0, 0.7, 1.3])
+    mask = np.zeros(nt, dtype=bool)
+    mask[5:20] = True
+    J, seed = R.mmf_costs(u, mask, kind, tau=7.0)
+    for trial in range(3):
+        du = rng.normal(size=(nt, m)) + 1j * rng.normal(size=(nt, m))
+        e = 1e-6
+        fd = (R.mmf_costs(u + e * du, mask, kind, 7.0)[0] - R.mmf_costs(u - e * du, mask, kind, 7.0)[0]) / (2 * e)
+        ad = 2 * np.real(np.sum(np.conj(seed) * du))
+        assert abs(fd - ad) <= 1e-7 * max(abs(fd), 1e-12)
+
+
+def test_mmf_costs_bounds_and_single_mode():
+    """mmf_cost.jl:112-114: max(r) - log(K)/tau <= J_worst <= max(r) over the K
+    modes with energy (a zero mode is skipped); fundamental = r_1; for one
+    mode the three costs coincide with the band fraction."""
+    rng = np.random.default_rng(12)
+    nt = 64
+    mask = np.zeros(nt, dtype=bool)
+    mask[40:60] = True
+    u = rng.normal(size=(nt, 4)) + 1j * rng.normal(size=(nt, 4))
+    u[:, 2] = 0
+    r = [np.sum(np.abs(u[mask, k]) ** 2) / np.sum(np.abs(u[:, k]) ** 2) for k in (0, 1, 3)]
+    J, seed = R.mmf_costs(u, mask, "worst_mode", tau=50.0)
+    assert max(r) - np.log(3) / 50 <= J <= max(r)
+    assert np.all(seed[:, 2] == 0)
+    assert math.isclose(R.mmf_costs(u, mask, "fundamental")[0], r[0], rel_tol=1e-14)
+    u1 = u[:, :1]
+    costs = [R.mmf_costs(u1, mask, k)[0] for k in ("sum", "fundamental", "worst_mode")]
+    assert max(costs) - min(costs) <= 1e-15
+    assert math.isclose(costs[0], R.band_cost(u1, mask)[0], rel_tol=1e-14)
