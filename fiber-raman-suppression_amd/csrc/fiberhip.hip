@@ -5,6 +5,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -576,13 +577,32 @@ struct GradBufs {
     double *hs, *zs;
     long long* stats;
     int* status;
-    int phase;  // SmArgs::phase
+    int phase;                // SmArgs::phase
+    void* work = nullptr;     // workgroup slices (null: the plan's)
+    bool fill_status = false; // clear status with a fill kernel instead of hipMemsetAsync
 };
 
 GradBufs plan_grad_bufs(fh_plan p)
 {
     return GradBufs{p->ckpt.p, p->tape_u.p, p->tape_h.p, (double*)p->hs.p, (double*)p->zs.p,
                     (long long*)p->sstats.p, (int*)p->status.p, 3};
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(int* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int clear_status(const GradBufs& gb, int batch, hipStream_t s)
+{
+    if (gb.fill_status) {
+        hipLaunchKernelGGL(k_zero_words, dim3((batch + 255) / 256), dim3(256), 0, s, gb.status, batch);
+        FH_HIP(hipGetLastError());
+    } else {
+        FH_HIP(hipMemsetAsync(gb.status, 0, sizeof(int) * (size_t)batch, s));
+    }
+    return FH_OK;
 }
 
 template <typename R>
@@ -603,7 +623,7 @@ int launch_gradient_with(fh_plan p, const GradBufs& gb, int batch, const void* d
         fh::MmArgs<R> m{};
         m.u0 = (const fh::C2<R>*)d_u0;
         m.out = (fh::C2<R>*)d_uL;
-        m.work = (fh::C2<R>*)p->work.p;
+        m.work = (fh::C2<R>*)(gb.work ? gb.work : p->work.p);
         m.disp = (const double*)p->disp.p;
         m.hn = (const fh::C2<R>*)p->hn.p;
         m.shn = (const R*)p->shn.p;
@@ -631,7 +651,7 @@ int launch_gradient_with(fh_plan p, const GradBufs& gb, int batch, const void* d
         m.ntaped = p->ntaped;
         m.lengths = d_lengths;
         m.nsteps = d_nsteps;  // each at most the plan's steps (checkpoint capacity)
-        if (gb.phase & 1) FH_HIP(hipMemsetAsync(gb.status, 0, sizeof(int) * (size_t)batch, s));
+        if ((gb.phase & 1) && clear_status(gb, batch, s)) return FH_EDEVICE;
         hipError_t e;
         if (p->adaptive) {
             m.h = 0.0;
@@ -664,8 +684,9 @@ int launch_gradient_with(fh_plan p, const GradBufs& gb, int batch, const void* d
     a.tape_h = (R*)gb.tape_h;
     a.status = gb.status;
     a.phase = gb.phase;
+    if (gb.work) a.work = (fh::C2<R>*)gb.work;
     if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
-    if (gb.phase & 1) FH_HIP(hipMemsetAsync(gb.status, 0, sizeof(int) * (size_t)batch, s));
+    if ((gb.phase & 1) && clear_status(gb, batch, s)) return FH_EDEVICE;
     fh::TsArgs<R> ta = ts_args<R>(p, a);
     ta.hs = gb.hs;
     ta.zs = gb.zs;
@@ -993,9 +1014,127 @@ int enqueue_batch(fh_plan p, int batch, bool grad, const double* d_lengths, cons
 
 }  // namespace fhi
 
+// ------------------------------------------- allocation-free gradient
+//
+// fh_gradient_device_noalloc keeps every buffer its kernels need beyond the
+// caller's inputs and outputs (workgroup slices, status words, checkpoints,
+// stage tape, adaptive step records, a device objective's seed) in this
+// fixed array instead of plan memory.  Each GPU holds its own copy for the
+// life of the process; fh_context_create resolves its device address once per
+// device.  In host code `scratch` names a host-side stand-in, so only the
+// resolved address is ever passed to kernels.
+constexpr size_t kNoallocScratchBytes = 256ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), resolved on first use.
+int resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(FH_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return FH_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(FH_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return FH_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Regions of the scratch array for `batch` solves of plan p, laid out like the
+// plan's own buffers (per-solve strides of its layout).
+struct ScratchLayout {
+    size_t status, work, ckpt, tape_u, tape_h, hs, zs, stats, seed, total;
+};
+
+ScratchLayout scratch_layout(fh_plan p, int batch)
+{
+    const size_t B = (size_t)batch, nt = (size_t)p->d.nt, M = (size_t)p->d.modes;
+    const size_t V = p->d.precision == FH_PREC_FP64 ? 16 : 8;
+    const size_t work = M > 1 ? p->grad_work_per_solve : p->work_per_solve;
+    const size_t steps = p->adaptive ? (size_t)p->max_steps : 0;
+    ScratchLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += align256(bytes);
+        return at;
+    };
+    L.status = take(sizeof(int) * B);
+    L.work = take(V * work * B);
+    L.ckpt = take(p->ckpt_per_solve * B);
+    L.tape_u = take(p->tape_u_per_solve * B);
+    L.tape_h = take(p->tape_h_per_solve * B);
+    L.hs = take(sizeof(double) * steps * B);
+    L.zs = take(sizeof(double) * steps * B);
+    L.stats = take(sizeof(long long) * 3 * B);
+    L.seed = take(M > 1 && p->d.objective != FH_OBJ_BAND_SUM ? V * nt * M * B : 0);
+    L.total = off;
+    return L;
+}
+
+template <typename R>
+int gradient_noalloc(fh_plan p, int batch, const void* d_u0, const void* d_seed, void* d_uL, void* d_lam0,
+                     double* d_grad, double* d_cost, hipStream_t s, unsigned char* base, const ScratchLayout& L)
+{
+    GradBufs gb{base + L.ckpt, base + L.tape_u, base + L.tape_h, (double*)(base + L.hs), (double*)(base + L.zs),
+                (long long*)(base + L.stats), (int*)(base + L.status), 3, base + L.work, true};
+    if (d_seed || p->d.modes == 1 || p->d.objective == FH_OBJ_BAND_SUM)
+        return launch_gradient_with<R>(p, gb, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s);
+    void* seed = base + L.seed;  // multimode objective other than the band sum: seed from the output
+    gb.phase = 1;
+    int rc = launch_gradient_with<R>(p, gb, batch, d_u0, seed, d_uL, d_lam0, d_grad, d_cost, s);
+    if (rc) return rc;
+    const double tau = p->d.objective_tau > 0 ? p->d.objective_tau : 50.0;
+    hipError_t e = fh::launch_objective(sizeof(R) == 8, d_uL, (const unsigned char*)p->mask.p, p->d.nt, p->d.modes,
+                                        p->d.objective, tau, p->d.log_cost, seed, d_cost, batch, s);
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("objective launch: ") + hipGetErrorString(e));
+    gb.phase = 2;
+    return launch_gradient_with<R>(p, gb, batch, d_u0, seed, d_uL, d_lam0, d_grad, d_cost, s);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+int fh_gradient_device_noalloc(fh_plan p, int32_t batch, const void* d_u0, const void* d_seed, void* d_uL,
+                               void* d_lam0, double* d_grad, double* d_cost, void* stream)
+{
+    if (!p || !d_u0 || !d_uL || !d_lam0 || !d_grad || !d_cost) return fail(FH_EINVAL, "NULL argument");
+    if (batch < 1 || batch > p->d.max_batch) return fail(FH_EINVAL, "batch must lie in [1, max_batch]");
+    if (!d_seed && !p->has_mask) return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
+    if (!p->grad_error.empty()) return fail(FH_EINVAL, p->grad_error);
+    if (p->d.modes > 1 && !p->gamma_full_sym)
+        return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
+    const int dev = p->ctx->device;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = dev >= 0 && dev < kMaxDevices ? g_scratch_ptr[dev] : nullptr;
+    }
+    if (!base) return fail(FH_EDEVICE, "scratch of this device is not resolved (create its context first)");
+    const ScratchLayout L = scratch_layout(p, batch);
+    if (L.total > kNoallocScratchBytes)
+        return fail(FH_EINVAL, "this gradient needs " + std::to_string(L.total) + " bytes of scratch for " +
+                                   std::to_string(batch) + " solves, above the " +
+                                   std::to_string(kNoallocScratchBytes) + " of the allocation-free entry point; "
+                                   "lower the batch, checkpoint_every or tape_segments");
+    FH_HIP(hipSetDevice(dev));
+    hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
+    unsigned char* b = (unsigned char*)base;
+    return p->d.precision == FH_PREC_FP64
+               ? gradient_noalloc<double>(p, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s, b, L)
+               : gradient_noalloc<float>(p, batch, d_u0, d_seed, d_uL, d_lam0, d_grad, d_cost, s, b, L);
+}
 
 int fh_context_create(int device, fh_context* out)
 {
@@ -1004,6 +1143,8 @@ int fh_context_create(int device, fh_context* out)
     FH_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(FH_EINVAL, "no such device");
     FH_HIP(hipSetDevice(device));
+    int rc = resolve_scratch(device);  // the allocation-free entry point's array on this device
+    if (rc) return rc;
     fh_context c = new fh_context_s;
     c->device = device;
     hipError_t e = hipStreamCreate(&c->stream);

@@ -161,6 +161,7 @@ _SIGNATURES = {
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
     "fh_gradient": ([_VP, ctypes.c_int32, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
     "fh_gradient_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "fh_gradient_device_noalloc": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP], ctypes.c_int),
     "fh_time_gradient_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, ctypes.c_int32,
                                  ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "fh_device_alloc": ([_VP, ctypes.c_int64, ctypes.POINTER(_VP)], ctypes.c_int),

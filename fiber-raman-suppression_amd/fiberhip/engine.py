@@ -307,6 +307,16 @@ class Plan:
             out = {k: (v[0] if isinstance(v, np.ndarray) else v) for k, v in out.items()}
         return out
 
+    def gradient_device(self, batch: int, d_u0: int, d_seed: int, d_uL: int, d_lam0: int, d_grad: int,
+                        d_cost: int, stream: int = 0, noalloc: bool = False) -> None:
+        """Enqueue a gradient on device-resident arrays (pointers as ints; d_seed
+        0 for the plan's objective).  noalloc: fh_gradient_device_noalloc, which
+        keeps its buffers in the library's fixed per-GPU scratch array."""
+        fn = lib().fh_gradient_device_noalloc if noalloc else lib().fh_gradient_device
+        check(fn(self.handle, int(batch), ctypes.c_void_p(d_u0), ctypes.c_void_p(d_seed) if d_seed else None,
+                 ctypes.c_void_p(d_uL), ctypes.c_void_p(d_lam0), ctypes.c_void_p(d_grad), ctypes.c_void_p(d_cost),
+                 ctypes.c_void_p(stream) if stream else None))
+
     def forward_tape(self, u0) -> "Tape":
         """Forward solve that keeps its tape on the device (fh_forward_tape);
         Tape.adjoint() then sweeps backward without a second forward solve."""

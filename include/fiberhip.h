@@ -204,6 +204,20 @@ int fh_gradient(fh_plan plan, int32_t batch, const double *u0, const double *see
 int fh_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, const void *d_seed, void *d_uL,
                        void *d_lam0, double *d_grad, double *d_cost, void *stream);
 
+/* fh_gradient_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The plan is created beforehand (plan creation allocates); the call itself
+ * keeps every buffer beyond its inputs and outputs (workgroup slices, status,
+ * checkpoints, stage tape, step records, a device objective's seed) in one
+ * fixed 256 MiB array per GPU, held until the process ends, and clears what
+ * must start at zero with a fill kernel.  A call whose solves need more
+ * (fh_plan_check sizes the parts) fails with FH_EINVAL.  Calls on one GPU
+ * share that array: calls that overlap on one GPU race, so keep them on one
+ * stream or otherwise ordered.  Results equal fh_gradient_device's. */
+int fh_gradient_device_noalloc(fh_plan plan, int32_t batch, const void *d_u0, const void *d_seed, void *d_uL,
+                               void *d_lam0, double *d_grad, double *d_cost, void *stream);
+
 /* Event-timed back-to-back gradient launches (see fh_time_forward_device). */
 int fh_time_gradient_device(fh_plan plan, int32_t batch, const void *d_u0, void *d_uL, void *d_lam0,
                             double *d_grad, double *d_cost, int32_t launches, float *ms_out);
