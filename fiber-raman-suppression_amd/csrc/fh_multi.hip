@@ -43,9 +43,7 @@ template <typename R> hipError_t by_size(bool grad, int nt, int m, const fh::MmA
     case 2048: return by_modes<R, 2048>(grad, m, a, batch, s);
     case 4096: return by_modes<R, 4096>(grad, m, a, batch, s);
     case 8192: return by_modes<R, 8192>(grad, m, a, batch, s);
-    case 16384:
-        if constexpr (sizeof(R) == 4) return by_modes<R, 16384>(grad, m, a, batch, s);
-        return hipErrorInvalidValue;
+    case 16384: return by_modes<R, 16384>(grad, m, a, batch, s);
     case 32768:
         if constexpr (sizeof(R) == 4) return by_modes<R, 32768>(grad, m, a, batch, s);
         return hipErrorInvalidValue;
@@ -62,7 +60,7 @@ bool multi_supported(int nt, int m, bool fp64)
     // FP64 with two modes is not offered: its kernel returned differing bits
     // for 1-2 of 16 identical solves on some launches (N = 4096 and 8192).
     const bool mok = (m == 2 && !fp64) || m == 3 || m == 4 || m == 6 || m == 10;
-    const bool nok = nt >= 256 && nt <= (fp64 ? 8192 : 32768);
+    const bool nok = nt >= 256 && nt <= (fp64 ? 16384 : 32768);
     return mok && nok;
 }
 

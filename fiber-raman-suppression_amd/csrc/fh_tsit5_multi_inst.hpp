@@ -33,9 +33,7 @@ template <typename R, int M, bool GRAD> hipError_t by_size(int nt, const MtArgs<
     case 2048: return go<R, 2048, M, GRAD>(a, batch, s);
     case 4096: return go<R, 4096, M, GRAD>(a, batch, s);
     case 8192: return go<R, 8192, M, GRAD>(a, batch, s);
-    case 16384:
-        if constexpr (sizeof(R) == 4) return go<R, 16384, M, GRAD>(a, batch, s);
-        return hipErrorInvalidValue;
+    case 16384: return go<R, 16384, M, GRAD>(a, batch, s);
     case 32768:
         if constexpr (sizeof(R) == 4) return go<R, 32768, M, GRAD>(a, batch, s);
         return hipErrorInvalidValue;

@@ -292,7 +292,7 @@ int check_desc(const fh_problem_desc* d)
         if (d->nt < 64 || d->nt > nmax)
             return fail(FH_EUNSUPPORTED, "nt must lie in [64, " + std::to_string(nmax) + "] for this precision");
     } else if (!fh::multi_supported(d->nt, d->modes, d->precision == FH_PREC_FP64)) {
-        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {3,4,6,10} with nt in [256, 8192] (FP64) and "
+        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {3,4,6,10} with nt in [256, 16384] (FP64) and "
                                      "M in {2,3,4,6,10} with nt in [256, 32768] (FP32)");
     }
     return FH_OK;

@@ -140,3 +140,24 @@ def test_grin6_full_fp32_vs_fp64(gpu_context):
     err = mode_energy_rel(out["fp32"], out["fp64"])
     print("grin6 full fp32 vs fp64 per-mode energy rel err:", err, flush=True)
     assert np.all(err <= 1e-5)
+
+
+@pytest.mark.parametrize("modes", [3, 6])
+def test_fp64_16384_multimode_matches_oracle(gpu_context, modes):
+    """FP64 multimode at 16384 samples (the serving trace's largest FP64
+    class): field within 1e-11 of the oracle, and four identical launches of
+    one batch return identical bits."""
+    import fiberhip
+    from fiberhip import grin
+
+    problem = grin.grin_problem(modes, nt=16384, time_window_ps=47.061, length_m=0.004)
+    steps = 8
+    print(f"case: fp64 multimode M={modes} nt=16384 steps={steps}", flush=True)
+    plan = fiberhip.Plan.from_problem(problem, steps=steps, precision="fp64", max_batch=4)
+    out, _ = plan.forward(np.stack([problem.uω0] * 4))
+    ref = _oracle(problem, steps)
+    print("fp64 16384 field rel", rel(out[0], ref), flush=True)
+    assert rel(out[0], ref) <= 1e-11
+    assert np.all(mode_energy_rel(out[0], ref) <= 1e-11)
+    for b in range(1, 4):
+        assert np.array_equal(out[b], out[0])
