@@ -1,4 +1,5 @@
-"""Parabolic GRIN multimode fibers with analytic Hermite-Gaussian modes.
+"""Parabolic GRIN multimode fibers: analytic Hermite-Gaussian modes, or the
+reference's finite-difference mode solve (end of this file).
 
 Builds the operators that get_disp_fiber_params / get_params produce
 (fibers.jl:91-132): the Kerr tensor gamma = S_K n2 w0 / c from the overlap
@@ -65,7 +66,7 @@ def hg_modes(modes: int, wavelength_um=1.55, radius_um=25.0, core_na=0.2, nx=256
     _, k, g = _profile(wavelength_um, radius_um, core_na)
     s = 1 / math.sqrt(k * g)
     dx = window_um / nx
-    x = np.arange(-nx // 2, nx // 2) * dx
+    x = np.arange(-nx / 2, nx / 2) * dx
     X, Y = np.meshgrid(x, x, indexing="ij")
     out = np.empty((nx, nx, modes))
     for i, (p, q) in enumerate(MODE_ORDER[:modes]):
@@ -109,17 +110,24 @@ def grin_dispersion(sim: dict, modes: int, beta_order=2, delta_f_thz=1.0, radius
 def grin_problem(modes: int = 6, nt: int = 4096, time_window_ps: float = 12.0, length_m: float = 0.3,
                  energy_j: float = 10e-9, fwhm_s: float = 100e-15, seed: int = 1, nx: int = 256,
                  window_um: float = 80.0, radius_um=25.0, core_na=0.2, wavelength_m=1550e-9, fR=0.18,
-                 launch=None) -> FiberFieldProblem:
-    """Default problem: parabolic GRIN (50 um core, NA 0.2), M HG modes on an
+                 launch=None, mode_solver: str = "hg") -> FiberFieldProblem:
+    """Default problem: parabolic GRIN (50 um core, NA 0.2), M modes on an
     nx^2 grid, sech^2 pulse of energy_j split over modes by Dirichlet(1)
     weights with uniform phases (seeded), initial_modes normalised
-    (physics_models.jl:289-297)."""
+    (physics_models.jl:289-297).  mode_solver "hg" uses the analytic
+    Hermite-Gaussian modes and beta_pq; "fd" the reference's procedure
+    (get_params: finite-difference eigensolve of the index profile)."""
     sim = get_disp_sim_params(wavelength_m, modes, nt, time_window_ps, 2)
-    phi, _, dx = hg_modes(modes, wavelength_m * 1e6, radius_um, core_na, nx, window_um)
-    sk = overlap_tensor(phi, dx * 1e-6)
     w0 = 2 * math.pi * sim["f0"] * 1e12
-    gamma = sk * N2_SILICA * w0 / 2.99792458e8
-    D = grin_dispersion(sim, modes, radius_um=radius_um, core_na=core_na)
+    if mode_solver == "fd":
+        _, D, gamma, phi, _ = get_params(sim["f0"], nx, window_um, radius_um, core_na, 2.0, modes, nt, sim["Δt"], 2)
+    elif mode_solver == "hg":
+        phi, _, dx = hg_modes(modes, wavelength_m * 1e6, radius_um, core_na, nx, window_um)
+        sk = overlap_tensor(phi, dx * 1e-6)
+        gamma = sk * N2_SILICA * w0 / 2.99792458e8
+        D = grin_dispersion(sim, modes, radius_um=radius_um, core_na=core_na)
+    else:
+        raise ValueError('mode_solver must be "hg" or "fd"')
     if launch is None:
         rng = np.random.default_rng(seed)
         w = rng.dirichlet(np.ones(modes))
@@ -132,4 +140,93 @@ def grin_problem(modes: int = 6, nt: int = 4096, time_window_ps: float = 12.0, l
     fiber = {"ϕ": phi, "Dω": D, "γ": gamma, "L": float(length_m), "zsave": None, "x": None,
              "gain_parameters": 0.0}
     fiber.update(raman_fields(sim, fR))
-    return fiber_field_problem(uw0, fiber, sim, preset="GRIN_50_HG")
+    return fiber_field_problem(uw0, fiber, sim, preset="GRIN_50_HG" if mode_solver == "hg" else "GRIN_50_FD")
+
+
+# ------------------------------------------------ finite-difference modes
+#
+# The reference's own GRIN setup (fibers.jl:28-54, 91-132, 159-197): the index
+# profile on an nx^2 grid, the scalar wave equation discretised with the
+# 5-point stencil, its most-bound eigenpairs, beta(f) on a symmetric frequency
+# stencil differentiated with central finite-difference weights, and the
+# overlap tensor of the modes at f0.  The eigenpairs are found by shift-invert
+# Lanczos about k^2 n_co^2 (above every bound mode), which returns the same
+# eigenpairs as the reference's largest-real-part Arnoldi; within a degenerate
+# group the basis is arbitrary, as there.
+
+
+def build_grin(wavelength_um: float, nx: int, window_um: float, radius_um: float, core_na: float,
+               alpha: float):
+    """build_GRIN, fibers.jl:28-54: eps = n(r)^2 with n = max(n_cl, n_co - (n_co - n_cl)(r/a)^alpha)
+    on x = (-nx/2 .. nx/2-1) dx; eps[i, j] at (x[j], y[i]) (meshgrid, helpers.jl:8-12)."""
+    nco = silica_index(wavelength_um)
+    ncl = math.sqrt(nco ** 2 - core_na ** 2)
+    dx = window_um / nx
+    x = np.arange(-nx / 2, nx / 2) * dx
+    X = np.tile(x[None, :], (nx, 1))
+    Y = np.tile(x[:, None], (1, nx))
+    eps = np.maximum(ncl, nco - (nco - ncl) * (np.sqrt(X ** 2 + Y ** 2) / radius_um) ** alpha) ** 2
+    return eps, x, dx
+
+
+def solve_for_fiber_modes(wavelength_um: float, nmodes: int, dx: float, eps: np.ndarray):
+    """solve_for_fiber_modes, fibers.jl:159-197: the nmodes largest eigenpairs of
+    A = laplacian_5pt + eps (2 pi / lambda)^2 (zero field outside the grid).
+    Returns beta^2 [um^-2] (descending), unit-norm modes (nx*nx, nmodes) in
+    column-major grid order, and n_eff = lambda sqrt(beta^2) / (2 pi)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    nx, ny = eps.shape
+    k2 = (2 * math.pi / wavelength_um) ** 2
+    c = 1.0 / dx ** 2
+    # column-major index i + nx j: neighbours along the first grid index differ by 1, along the second by nx
+    d0 = eps.reshape(-1, order="F") * k2 - 4 * c
+    ex = np.full(nx * ny - 1, c)
+    ex[np.arange(1, nx * ny) % nx == 0] = 0.0  # no coupling across columns
+    ey = np.full(nx * ny - nx, c)
+    A = sp.diags([d0, ex, ex, ey, ey], [0, 1, -1, nx, -nx], format="csc")
+    sigma = float(np.max(eps)) * k2
+    d, v = sla.eigsh(A, k=nmodes, sigma=sigma, which="LM", tol=1e-14)
+    order = np.argsort(d)[::-1]
+    d, v = d[order], v[:, order]
+    v = v / np.sqrt(np.sum(v ** 2, axis=0))
+    return d, v, wavelength_um * np.sqrt(d) / (2 * math.pi)
+
+
+def central_fd_weights(points: int, order: int) -> np.ndarray:
+    """Weights w on the offsets -(points-1)/2 .. (points-1)/2 with
+    sum_j w_j o_j^m / m! = [m == order] for m < points (central_fdm(points, order).coefs)."""
+    half = (points - 1) // 2
+    o = np.arange(-half, half + 1, dtype=float)
+    V = np.array([o ** m / math.factorial(m) for m in range(points)])
+    rhs = np.zeros(points)
+    rhs[order] = 1.0
+    return np.linalg.solve(V, rhs)
+
+
+def get_params(f0_thz: float, nx: int, window_um: float, radius_um: float, core_na: float, alpha: float, modes: int,
+               nt: int, dt_ps: float, beta_order: int, delta_f_thz: float = 1.0):
+    """get_params, fibers.jl:91-132: (betas (beta_order+1, M), D (nt, M) rad/m,
+    gamma (M,M,M,M) 1/(W m), modes (nx, nx, M), x) from finite-difference modes."""
+    c0 = 2.99792458e8
+    points = 2 * beta_order + 1
+    half = (points - 1) // 2
+    beta_f = np.zeros((points, modes))
+    for i, off in enumerate(range(-half, half + 1)):
+        lam_um = c0 / ((f0_thz + delta_f_thz * off) * 1e12) * 1e6
+        eps, _, dx = build_grin(lam_um, nx, window_um, radius_um, core_na, alpha)
+        _, _, neff = solve_for_fiber_modes(lam_um, modes, dx, eps)
+        beta_f[i] = 2 * math.pi * neff / (lam_um * 1e-6)
+    dbeta = np.stack([central_fd_weights(points, n) @ beta_f / (2 * math.pi * delta_f_thz / 1e-12) ** n
+                      for n in range(1, beta_order + 1)])
+    betas = np.vstack([beta_f[half] - beta_f[half, 0], dbeta[0] - dbeta[0, 0], dbeta[1:]])
+    w = 2 * math.pi * fftfreq(nt, 1 / dt_ps) * 1e12
+    D = np.stack([w ** n / math.factorial(n) for n in range(beta_order + 1)], axis=1) @ betas
+    lam0_um = c0 / f0_thz / 1e12 * 1e6
+    eps, x, dx = build_grin(lam0_um, nx, window_um, radius_um, core_na, alpha)
+    _, phi, _ = solve_for_fiber_modes(lam0_um, modes, dx, eps)
+    phi = phi.reshape(nx, nx, modes, order="F")
+    sk = overlap_tensor(phi, dx * 1e-6)
+    gamma = sk * N2_SILICA * 2 * math.pi * f0_thz * 1e12 / c0
+    return betas, D, gamma, phi, x

@@ -331,3 +331,49 @@ def test_adjoint_model_control_fields():
         m.fields({"energy": -1.0})
     f = m.fields({"phase": np.zeros(256), "energy": 4.0})
     assert f["alpha"] == 2.0 and f["phase_shape"] == "vector"
+
+
+def test_fd_grin_modes_match_hermite_gaussian_closed_forms():
+    """The finite-difference GRIN setup (build_GRIN / solve_for_fiber_modes /
+    get_params, fibers.jl:28-54, 91-132, 159-197) against the parabolic
+    profile's closed forms: n_eff of the ten lowest modes within the grid error of
+    beta_pq / k (2.5e-5 at 128 points), each degenerate group spanning the Hermite-Gaussian one
+    (singular values of the projection > 0.99998), S_0000 and D(w) within the
+    grid's O(dx^2) of the analytic builder, and the discretisation error
+    shrinking like dx^2 from 101 to 128 points."""
+    from fiberhip import grin
+    from fiberhip.problem import get_disp_sim_params
+
+    eps, x, dx = grin.build_grin(1.55, 128, 80.0, 25.0, 0.2, 2.0)
+    assert eps.shape == (128, 128) and x[64] == 0.0
+    _, v, neff = grin.solve_for_fiber_modes(1.55, 10, dx, eps)
+    nco, k, g = grin._profile(1.55, 25.0, 0.2)
+    hg_neff = np.array([math.sqrt(k * k * nco * nco - 2 * k * g * (p + q + 1)) / k for p, q in grin.MODE_ORDER])
+    assert np.all(np.abs(neff - hg_neff) <= 2.5e-5)  # O(dx^2): 1.4e-6 (fundamental) to 1.8e-5 (p + q = 3)
+    sim = get_disp_sim_params(1550e-9, 6, 1024, 4.0, 2)
+    Dh = grin.grin_dispersion(sim, 6)
+    errs = []
+    for nx in (101, 128):
+        _, D, gamma, phi, _ = grin.get_params(sim["f0"], nx, 80.0, 25.0, 0.2, 2.0, 6, 1024, sim["Δt"], 2)
+        hg, _, dxx = grin.hg_modes(6, 1.55, 25.0, 0.2, nx, 80.0)
+        f, h = phi.reshape(-1, 6), hg.reshape(-1, 6)
+        for grp in ([0], [1, 2], [3, 4, 5]):
+            assert np.linalg.svd(f[:, grp].T @ h[:, grp])[1].min() > 0.99998
+        s_fd = grin.overlap_tensor(phi, dxx * 1e-6)[0, 0, 0, 0]
+        s_hg = grin.overlap_tensor(hg, dxx * 1e-6)[0, 0, 0, 0]
+        errs.append((abs(s_fd / s_hg - 1), np.abs(D - Dh).max() / np.abs(Dh).max()))
+        # gamma is fully permutation symmetric and D is referenced to mode 1
+        assert np.allclose(gamma, np.transpose(gamma, (1, 0, 2, 3)), rtol=1e-13)
+        assert np.allclose(gamma, np.transpose(gamma, (0, 2, 1, 3)), rtol=1e-13)
+        assert D[0, 0] == 0.0
+    assert errs[1][0] < 2e-3 and errs[1][1] < 3e-3
+    ratio = (128 / 101) ** 2
+    assert errs[0][0] / errs[1][0] > 0.8 * ratio and errs[0][1] / errs[1][1] > 0.8 * ratio
+
+
+def test_central_fd_weights():
+    from fiberhip import grin
+
+    assert np.allclose(grin.central_fd_weights(5, 1), [1 / 12, -2 / 3, 0, 2 / 3, -1 / 12], atol=1e-15)
+    assert np.allclose(grin.central_fd_weights(5, 2), [-1 / 12, 4 / 3, -5 / 2, 4 / 3, -1 / 12], atol=1e-14)
+    assert np.allclose(grin.central_fd_weights(3, 1), [-0.5, 0, 0.5], atol=1e-15)
