@@ -9,6 +9,15 @@ template <typename R> struct MmArgs;
 // Workgroup size per precision and transform length: 8 samples per thread
 // for FP64 (1024 threads from 8192 samples up) keeps the radix-8 working set
 // in registers; FP32 uses 16 from 4096 samples up.
+// Units that instantiate multimode kernels are built once per precision
+// (FH_MM_PART = 32 or 64); without the define a unit builds both.  The FP64
+// part is compiled with every memory wait forced to zero: without it the
+// FP64 multimode kernels returned results that differed between identical
+// solves of one launch.
+#ifndef FH_MM_PART
+#define FH_MM_PART 0
+#endif
+
 template <typename R, int N> struct SingleShape {
     static constexpr int T = sizeof(R) == 8 ? (N >= 8192 ? 1024 : (N >= 256 ? N / 8 : N / 4))
                                             : (N >= 32768 ? 1024 : (N >= 4096 ? N / 16 : (N >= 512 ? N / 8 : N / 4)));
@@ -40,14 +49,14 @@ hipError_t launch_field_metrics(int nt, bool fp64, const void* spec, const void*
                                 int modes, double* out, int count, hipStream_t s);
 
 template <typename R> struct MtArgs;
-int tsit5_multi_fields(int m);
-int tsit5_multi_grad_fields(int m);
+int tsit5_multi_fields(int nt, int m, bool fp64);
+int tsit5_multi_grad_fields(int nt, int m, bool fp64);
 hipError_t launch_tsit5_multi(int nt, int m, bool grad, const MtArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_tsit5_multi(int nt, int m, bool grad, const MtArgs<float>& a, int batch, hipStream_t s);
 
 bool multi_supported(int nt, int m, bool fp64);
-int multi_fields(int m);
-int multi_grad_fields(int m);
+int multi_fields(int nt, int m, bool fp64);
+int multi_grad_fields(int nt, int m, bool fp64);
 hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s);
 hipError_t launch_multi_grad(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s);

@@ -10,8 +10,10 @@ namespace {
 
 template <typename R, int N, int M> hipError_t go(bool grad, const fh::MmArgs<R>& a, int batch, hipStream_t s)
 {
-    if constexpr (sizeof(R) == 8 && M == 2) {
-        return hipErrorInvalidValue;  // not offered (multi_supported)
+    if constexpr (M == 1 && !fh::mm_split<R, N>()) {
+        // one mode runs here only where the single-mode kernels have no
+        // shape (split transforms)
+        return hipErrorInvalidValue;
     } else {
         constexpr int T = fh::MultiShape<R, N>::T;
         if (grad)
@@ -25,6 +27,7 @@ template <typename R, int N, int M> hipError_t go(bool grad, const fh::MmArgs<R>
 template <typename R, int N> hipError_t by_modes(bool grad, int m, const fh::MmArgs<R>& a, int batch, hipStream_t s)
 {
     switch (m) {
+    case 1: return go<R, N, 1>(grad, a, batch, s);
     case 2: return go<R, N, 2>(grad, a, batch, s);
     case 3: return go<R, N, 3>(grad, a, batch, s);
     case 4: return go<R, N, 4>(grad, a, batch, s);
@@ -44,9 +47,7 @@ template <typename R> hipError_t by_size(bool grad, int nt, int m, const fh::MmA
     case 4096: return by_modes<R, 4096>(grad, m, a, batch, s);
     case 8192: return by_modes<R, 8192>(grad, m, a, batch, s);
     case 16384: return by_modes<R, 16384>(grad, m, a, batch, s);
-    case 32768:
-        if constexpr (sizeof(R) == 4) return by_modes<R, 32768>(grad, m, a, batch, s);
-        return hipErrorInvalidValue;
+    case 32768: return by_modes<R, 32768>(grad, m, a, batch, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -55,22 +56,22 @@ template <typename R> hipError_t by_size(bool grad, int nt, int m, const fh::MmA
 
 namespace fh {
 
+#if FH_MM_PART != 64
 bool multi_supported(int nt, int m, bool fp64)
 {
-    // FP64 with two modes is not offered: its kernel returned differing bits
-    // for 1-2 of 16 identical solves on some launches (N = 4096 and 8192).
-    const bool mok = (m == 2 && !fp64) || m == 3 || m == 4 || m == 6 || m == 10;
-    const bool nok = nt >= 256 && nt <= (fp64 ? 16384 : 32768);
+    // One mode runs on these kernels only where transforms are split.
+    const bool mok = m == 2 || m == 3 || m == 4 || m == 6 || m == 10 || (m == 1 && mm_split_rt(nt, fp64));
+    const bool nok = nt >= 256 && nt <= 32768 && (nt & (nt - 1)) == 0;
     return mok && nok;
 }
 
-int multi_fields(int m) { return 4 * m + (m * m + 1) / 2; }
+// Slice fields of the fixed-step multimode kernels (MmTraits::FIELDS,
+// FIELDS_GRAD), including the scratch of split transforms.
+int multi_fields(int nt, int m, bool fp64) { return 4 * m + (m * m + 1) / 2 + (mm_split_rt(nt, fp64) ? 2 : 0); }
 
-int multi_grad_fields(int m) { return 9 * m + 2 * ((m * m + 1) / 2); }
-
-hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
+int multi_grad_fields(int nt, int m, bool fp64)
 {
-    return by_size<double>(false, nt, m, a, batch, s);
+    return 9 * m + 2 * ((m * m + 1) / 2) + (mm_split_rt(nt, fp64) ? 2 : 0);
 }
 
 hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s)
@@ -78,14 +79,22 @@ hipError_t launch_multi(int nt, int m, const MmArgs<float>& a, int batch, hipStr
     return by_size<float>(false, nt, m, a, batch, s);
 }
 
-hipError_t launch_multi_grad(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
-{
-    return by_size<double>(true, nt, m, a, batch, s);
-}
-
 hipError_t launch_multi_grad(int nt, int m, const MmArgs<float>& a, int batch, hipStream_t s)
 {
     return by_size<float>(true, nt, m, a, batch, s);
 }
+#endif
+
+#if FH_MM_PART != 32
+hipError_t launch_multi(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
+{
+    return by_size<double>(false, nt, m, a, batch, s);
+}
+
+hipError_t launch_multi_grad(int nt, int m, const MmArgs<double>& a, int batch, hipStream_t s)
+{
+    return by_size<double>(true, nt, m, a, batch, s);
+}
+#endif
 
 }  // namespace fh

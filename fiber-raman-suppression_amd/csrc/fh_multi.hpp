@@ -71,15 +71,27 @@ template <typename R> struct MmArgs {
     int phase;   // gradient kernels: 1 forward sweep, 2 seed and backward sweep, 3 both (SmArgs::phase)
 };
 
+// Multimode fields transformed in two halves (mm_xform): FP64 with more than
+// 16 samples per thread at the 1024-thread workgroup shape.
+template <typename R, int N> constexpr bool mm_split() { return sizeof(R) == 8 && N > 16384; }
+inline bool mm_split_rt(int nt, bool fp64) { return fp64 && nt > 16384; }
+
 template <typename R, int N, int T, int M> struct MmTraits {
-    static constexpr int P = N / T;
+    static constexpr int P = N / T;  // field elements per thread
+    // FP64 fields of more than 16 samples per thread are transformed in two
+    // halves (mm_xform): NX-point block transforms of PV registers each
+    static constexpr bool SPLIT = mm_split<R, N>();
+    static constexpr int NX = SPLIT ? N / 2 : N;
+    static constexpr int PV = NX / T;
+    static constexpr int XTRA = SPLIT ? 2 : 0;  // scratch fields of split transforms
     static constexpr int NP = M * (M + 1) / 2;  // input pairs k <= l
-    static constexpr bool PLANAR = (size_t)LdsSize<N>::value * 2 * sizeof(R) > LdsBudget<R>::bytes;
-    static constexpr size_t LDS_BYTES = (size_t)LdsSize<N>::value * (PLANAR ? sizeof(R) : 2 * sizeof(R));
+    static constexpr bool PLANAR = (size_t)LdsSize<NX>::value * 2 * sizeof(R) > LdsBudget<R>::bytes;
+    static constexpr size_t LDS_BYTES = (size_t)LdsSize<NX>::value * (PLANAR ? sizeof(R) : 2 * sizeof(R));
     static constexpr int QMAX = M * M;
     static constexpr int QCMAX = (QMAX + 1) / 2;
-    // forward slice: y[M], st[M], ac[M], u[M], hq[QC]
-    static constexpr int FIELDS = 4 * M + QCMAX;
+    // forward slice: y[M], st[M], ac[M], u[M], hq[QC], split scratch[XTRA]
+    static constexpr int XA = 4 * M + QCMAX;
+    static constexpr int FIELDS = 4 * M + QCMAX + XTRA;
     // gradient slice adds lam[M], lacc[M], gx[M], G[M], gu[M], bq[QC]
     static constexpr int LAM = FIELDS, LACC = LAM + M, GX = LACC + M, GG = GX + M, GU = GG + M, BQ = GU + M;
     static constexpr int FIELDS_GRAD = BQ + QCMAX;
@@ -112,15 +124,137 @@ template <int M> struct InputPairs {
     }
 };
 
-// Per-workgroup views shared by the stage and pullback routines.
+// Per-workgroup views shared by the stage and pullback routines.  xa: the
+// first of the two slice fields split transforms use as scratch.
 template <typename R, int N, int T, int M> struct MmCtx {
     using V = C2<R>;
     const MmArgs<R>& a;
     const Buf& sl;
     const Buf &disp, &hn, &shn, &tw;
     int Q, QC;
+    int xa;
     __device__ __forceinline__ Fld fld(int f) const { return Fld{sl, (uint32_t)f * (uint32_t)(N * sizeof(V))}; }
 };
+
+// Entries of the twiddle table of a multimode plan: the block transform's,
+// and for split transforms the N/2 radix-2 twiddles w^n after them.
+template <typename R, int N, int T> constexpr int mm_twiddle_entries()
+{
+    return mm_split<R, N>() ? FftPlan<N / 2, T>::TW_SIZE + N / 2 : FftPlan<N, T>::TW_SIZE;
+}
+
+// Block transform of one field-sized signal: this thread's elements k < P are
+// v_k = pre(k), transformed (DIR -1: exp(-2 pi i n k / N), +1: its
+// conjugate, both unnormalised), and handed to post(k, x_k).  Element k is
+// index t + T k on both sides.
+//
+// Split transforms (FP64 with 32 samples per thread): the N-point transform
+// is one radix-2 step that stays inside each thread and two N/2-point block
+// transforms of P/2 registers each.  The -1 transform then takes its input in
+// natural order (element k is index t + T k) and returns its output in split
+// order (element k < P/2 is index 2 (t + T k), element k + P/2 the index after
+// it); the +1 transform takes split order and returns natural order:
+//   -1:  a_n = x_n + x_{n+N/2},  b_n = (x_n - x_{n+N/2}) w^n,  w = exp(-2 pi i / N),
+//        X_2m = FFT(a)_m,  X_2m+1 = FFT(b)_m
+//   +1:  e = IFFT(X_2m),  o = IFFT(X_2m+1),  x_n = e_n + w^-n o_n,  x_{n+N/2} = e_n - w^-n o_n
+// The state transforms map the frequency-domain fields (-1 input, +1 output)
+// to the time-domain fields, so frequency-domain fields, their tables and the
+// caller's arrays keep natural order and time-domain fields are in split
+// order (pointwise work there does not depend on the order; mm_convolve
+// handles the Raman convolution).  Half of the signal waits in scratch field
+// c.xa meanwhile.  Every pre(k) is read before any post(k) is written, so a
+// transform may work in place.
+template <typename R, int N, int T, int M, int DIR, class Pre, class Post>
+__device__ __forceinline__ void mm_xform(const MmCtx<R, N, T, M>& c, Pre&& pre, Post&& post, C2<R>* v, void* lds)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    constexpr int P = TR::P;
+    if constexpr (!TR::SPLIT) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = pre(k);
+        block_fft<R, N, T, DIR, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) post(k, v[k]);
+    } else {
+        using CV = Cyc<V, T>;
+        constexpr int PH = P / 2, NH = N / 2;
+        const uint32_t vo = CV::voff();
+        const uint32_t wb = vo + (uint32_t)(FftPlan<NH, T>::TW_SIZE * sizeof(V));
+        const Fld xs = c.fld(c.xa);
+        if constexpr (DIR < 0) {
+#pragma unroll
+            for (int j = 0; j < PH; ++j) {
+                const V a = pre(j), b = pre(j + PH);
+                v[j] = cadd(a, b);
+                xs.st(cmul(csub(a, b), c.tw.template ld<V>(wb, CV::soff(j))), vo, CV::soff(j));
+            }
+            block_fft<R, NH, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+            for (int j = 0; j < PH; ++j) post(j, v[j]);
+            drain_memory();  // re-reads the odd half from scratch
+#pragma unroll
+            for (int j = 0; j < PH; ++j) v[j] = xs.template ld<V>(vo, CV::soff(j));
+            block_fft<R, NH, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+            for (int j = 0; j < PH; ++j) post(j + PH, v[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < PH; ++j) v[j] = pre(j);
+            block_fft<R, NH, T, +1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+            for (int j = 0; j < PH; ++j) xs.st(v[j], vo, CV::soff(j));
+#pragma unroll
+            for (int j = 0; j < PH; ++j) v[j] = pre(j + PH);
+            block_fft<R, NH, T, +1, TR::PLANAR>(v, c.tw, lds);
+            drain_memory();  // re-reads the even half from scratch
+#pragma unroll
+            for (int j = 0; j < PH; ++j) {
+                const V e = xs.template ld<V>(vo, CV::soff(j));
+                const V o = cmul(v[j], cconj(c.tw.template ld<V>(wb, CV::soff(j))));
+                post(j, cadd(e, o));
+                post(j + PH, csub(e, o));
+            }
+        }
+    }
+}
+
+// Raman convolution of one packed pair: pre(k) -> -1 transform -> times hn
+// (conj(hn) with CONJ) -> +1 transform -> post(k, x_k).  With split
+// transforms the time-domain input is in split order, so the convolution runs
+// the other way round: +1 transform (split in, natural out), times the plan's
+// index-reversed table hn[-k], -1 transform (natural in, split out), which is
+// the same circular convolution; the spectrum waits in scratch field c.xa + 1
+// between the two.
+template <typename R, int N, int T, int M, bool CONJ, class Pre, class Post>
+__device__ __forceinline__ void mm_convolve(const MmCtx<R, N, T, M>& c, Pre&& pre, Post&& post, C2<R>* v,
+                                            void* lds)
+{
+    using V = C2<R>;
+    using TR = MmTraits<R, N, T, M>;
+    using CV = Cyc<V, T>;
+    constexpr int P = TR::P;
+    const uint32_t vo = CV::voff();
+    auto H = [&](int k) {
+        const V h = c.hn.template ld<V>(vo, CV::soff(k));
+        return CONJ ? cconj(h) : h;
+    };
+    if constexpr (!TR::SPLIT) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = pre(k);
+        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[k] = cmul(v[k], H(k));
+        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
+#pragma unroll
+        for (int k = 0; k < P; ++k) post(k, v[k]);
+    } else {
+        const Fld xb = c.fld(c.xa + 1);
+        mm_xform<R, N, T, M, +1>(c, pre, [&](int k, V x) { xb.st(cmul(x, H(k)), vo, CV::soff(k)); }, v, lds);
+        drain_memory();  // re-reads the spectrum
+        mm_xform<R, N, T, M, -1>(c, [&](int k) { return xb.template ld<V>(vo, CV::soff(k)); }, post, v, lds);
+    }
+}
 
 typedef float fh_floatx16 __attribute__((ext_vector_type(16)));
 
@@ -215,21 +349,20 @@ __device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double z
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
         const Fld y = c.fld(IN + m), u = c.fld(U + m);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
-            const V x = y.template ld<V>(vo, CV::soff(k));
-            v[k] = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
-        }
-        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) u.st(v[k], vo, CV::soff(k));
-        if constexpr (TAPE) {
-            const Fld tu{tape_u, (tslot * (uint32_t)M + (uint32_t)m) * FB};
-#pragma unroll
-            for (int k = 0; k < P; ++k) tu.st(v[k], vo, CV::soff(k));
-        }
+        const Fld tu{tape_u, (tslot * (uint32_t)M + (uint32_t)m) * FB};
+        mm_xform<R, N, T, M, -1>(
+            c,
+            [&](int k) {
+                R cs, sn;
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
+                const V x = y.template ld<V>(vo, CV::soff(k));
+                return V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
+            },
+            [&](int k, V x) {
+                u.st(x, vo, CV::soff(k));
+                if constexpr (TAPE) tu.st(x, vo, CV::soff(k));
+            },
+            v, lds);
     }
     drain_memory();  // re-reads u of step 1
     // 2. Kerr contraction per sample: dK_q = sum_p Gr[q][p] B_p (matrix cores
@@ -272,20 +405,16 @@ __device__ __forceinline__ void mm_rhs_head(const MmCtx<R, N, T, M>& c, double z
 #pragma unroll 1
     for (int cc = 0; cc < QC; ++cc) {
         const Fld hq = c.fld(HQ + cc);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = hq.template ld<V>(vo, CV::soff(k));
-        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = cmul(v[k], c.hn.template ld<V>(vo, CV::soff(k)));
-        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
         const Fld th{tape_h, (tslot * (uint32_t)QC + (uint32_t)cc) * FB};
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const V d = hq.template ld<V>(vo, CV::soff(k));
-            const V hv = V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y};
-            hq.st(hv, vo, CV::soff(k));
-            if constexpr (TAPE) th.st(hv, vo, CV::soff(k));
-        }
+        mm_convolve<R, N, T, M, false>(
+            c, [&](int k) { return hq.template ld<V>(vo, CV::soff(k)); },
+            [&](int k, V x) {
+                const V d = hq.template ld<V>(vo, CV::soff(k));
+                const V hv = V{a.one_m_fr * d.x + x.x, a.one_m_fr * d.y + x.y};
+                hq.st(hv, vo, CV::soff(k));
+                if constexpr (TAPE) th.st(hv, vo, CV::soff(k));
+            },
+            v, lds);
     }
 }
 
@@ -354,37 +483,36 @@ __device__ __forceinline__ void mm_stage(const MmCtx<R, N, T, M>& c, int s, doub
 #pragma unroll 1
     for (int i = 0; i < M; ++i) {
         const Fld eta = c.fld(3 * M + i);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = eta.template ld<V>(vo, CV::soff(k));
-        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
         const Fld y = c.fld(i), st = c.fld(M + i), ac = c.fld(2 * M + i);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
-            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
-            const V e = V{v[k].x * sc, v[k].y * sc};
-            const V kk = V{sn * e.x - cs * e.y, sn * e.y + cs * e.x};
-            const V x = st.template ld<V>(vo, CV::soff(k));
-            V cc = s != 0 ? ac.template ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
-            V yn;
-            if (s == 0) {
-                cc = kk;
-                yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-            } else if (s == 1) {
-                cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
-                yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
-            } else if (s == 2) {
-                cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
-                yn = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
-            } else {
-                const V t = V{cc.x + kk.x, cc.y + kk.y};
-                yn = V{x.x + h6 * t.x, x.y + h6 * t.y};
-                st.st(yn, vo, CV::soff(k));
-            }
-            if (s != 3) ac.st(cc, vo, CV::soff(k));
-            y.st(yn, vo, CV::soff(k));
-        }
+        mm_xform<R, N, T, M, +1>(
+            c, [&](int k) { return eta.template ld<V>(vo, CV::soff(k)); },
+            [&](int k, V w) {
+                R cs, sn;
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+                const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+                const V e = V{w.x * sc, w.y * sc};
+                const V kk = V{sn * e.x - cs * e.y, sn * e.y + cs * e.x};
+                const V x = st.template ld<V>(vo, CV::soff(k));
+                V cc = s != 0 ? ac.template ld<V>(vo, CV::soff(k)) : V{(R)0, (R)0};
+                V yn;
+                if (s == 0) {
+                    cc = kk;
+                    yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+                } else if (s == 1) {
+                    cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
+                    yn = V{x.x + h2 * kk.x, x.y + h2 * kk.y};
+                } else if (s == 2) {
+                    cc = V{cc.x + 2 * kk.x, cc.y + 2 * kk.y};
+                    yn = V{x.x + h1 * kk.x, x.y + h1 * kk.y};
+                } else {
+                    const V t = V{cc.x + kk.x, cc.y + kk.y};
+                    yn = V{x.x + h6 * t.x, x.y + h6 * t.y};
+                    st.st(yn, vo, CV::soff(k));
+                }
+                if (s != 3) ac.st(cc, vo, CV::soff(k));
+                y.st(yn, vo, CV::soff(k));
+            },
+            v, lds);
     }
 }
 
@@ -411,18 +539,17 @@ __device__ __forceinline__ void mm_rhs(const MmCtx<R, N, T, M>& c, double zs, in
 #pragma unroll 1
     for (int i = 0; i < M; ++i) {
         const Fld eta = c.fld(U + i);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = eta.template ld<V>(vo, CV::soff(k));
-        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
         const Fld out = c.fld(OUT + i);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
-            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
-            const V e = V{v[k].x * sc, v[k].y * sc};
-            out.st(V{sn * e.x - cs * e.y, sn * e.y + cs * e.x}, vo, CV::soff(k));
-        }
+        mm_xform<R, N, T, M, +1>(
+            c, [&](int k) { return eta.template ld<V>(vo, CV::soff(k)); },
+            [&](int k, V w) {
+                R cs, sn;
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+                const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+                const V e = V{w.x * sc, w.y * sc};
+                out.st(V{sn * e.x - cs * e.y, sn * e.y + cs * e.x}, vo, CV::soff(k));
+            },
+            v, lds);
     }
 }
 
@@ -442,9 +569,9 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     V* slice = a.work + sol * (size_t)TR::FIELDS * N;
     const Buf sl(slice, (uint32_t)((size_t)TR::FIELDS * N * sizeof(V)));
     const Buf disp(a.disp, M * N * 8), hn(a.hn, N * sizeof(V)), shn(a.shn, N * sizeof(R));
-    const Buf tw(a.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf tw(a.tw, mm_twiddle_entries<R, N, T>() * sizeof(V));
     const Buf in(a.u0 + sol * M * N, M * N * sizeof(V));
-    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, a.q, (a.q + 1) / 2};
+    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, a.q, (a.q + 1) / 2, TR::XA};
     const uint32_t vo = CV::voff();
 
     // y = st = u0
@@ -452,7 +579,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
         const Fld y = c.fld(m), st = c.fld(M + m);
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            const V x = in.ld<V>(CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
             y.st(x, vo, CV::soff(k));
             st.st(x, vo, CV::soff(k));
         }
@@ -460,7 +587,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
     const double Lb = a.lengths ? a.lengths[sol] : a.length;
     const int Sb = a.nsteps ? a.nsteps[sol] : a.steps;
     const double h = a.lengths ? Lb / Sb : a.h;
-    V v[P];
+    V v[TR::PV];
     const Buf none(a.u0, 0);
     for (int step = 0; step < Sb; ++step) {
         const double z = step * h;
@@ -482,7 +609,7 @@ __global__ __launch_bounds__(T) void k_rk4ip_multi(MmArgs<R> a)
             const V x = st.template ld<V>(vo, CV::soff(k));
             const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
             bad |= !(isfinite(o.x) && isfinite(o.y));
-            out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            out.st(o, CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
         }
     }
     if (bad) atomicOr(a.status + sol, 1);
@@ -518,24 +645,23 @@ __device__ __forceinline__ void mm_vjp(const MmCtx<R, N, T, M>& c, double zs, R 
 #pragma unroll 1
     for (int i = 0; i < M; ++i) {
         const Fld lam = c.fld(TR::LAM + i), gx = c.fld(TR::GX + i), G = c.fld(TR::GG + i);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const V l = lam.template ld<V>(vo, CV::soff(k));
-            V g = V{cl * l.x, cl * l.y};
-            if (cg != (R)0) {
-                const V x = gx.template ld<V>(vo, CV::soff(k));
-                g = V{g.x + cg * x.x, g.y + cg * x.y};
-            }
-            R cs, sn;
-            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
-            const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
-            const V e = V{g.x * sc, g.y * sc};
-            const V w = V{e.x * cs - e.y * sn, e.x * sn + e.y * cs};
-            v[k] = V{w.y, -w.x};  // -i w
-        }
-        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) G.st(v[k], vo, CV::soff(k));
+        mm_xform<R, N, T, M, -1>(
+            c,
+            [&](int k) {
+                const V l = lam.template ld<V>(vo, CV::soff(k));
+                V g = V{cl * l.x, cl * l.y};
+                if (cg != (R)0) {
+                    const V x = gx.template ld<V>(vo, CV::soff(k));
+                    g = V{g.x + cg * x.x, g.y + cg * x.y};
+                }
+                R cs, sn;
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(i * N * 8) + CD::soff(k)), zs, cs, sn);
+                const R sc = c.shn.template ld<R>(CR::voff(), CR::soff(k));
+                const V e = V{g.x * sc, g.y * sc};
+                const V w = V{e.x * cs - e.y * sn, e.x * sn + e.y * cs};
+                return V{w.y, -w.x};  // -i w
+            },
+            [&](int k, V x) { G.st(x, vo, CV::soff(k)); }, v, lds);
     }
     drain_memory();  // re-reads G
     // V2: gu_j = sum_i h_ij G_i; A_q in output-pair order, packed into bq
@@ -599,17 +725,13 @@ __device__ __forceinline__ void mm_vjp(const MmCtx<R, N, T, M>& c, double zs, R 
 #pragma unroll 1
     for (int cc = 0; cc < QC; ++cc) {
         const Fld bq = c.fld(TR::BQ + cc);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = bq.template ld<V>(vo, CV::soff(k));
-        block_fft<R, N, T, -1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = cmul(v[k], cconj(c.hn.template ld<V>(vo, CV::soff(k))));
-        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const V d = bq.template ld<V>(vo, CV::soff(k));
-            bq.st(V{a.one_m_fr * d.x + v[k].x, a.one_m_fr * d.y + v[k].y}, vo, CV::soff(k));
-        }
+        mm_convolve<R, N, T, M, true>(
+            c, [&](int k) { return bq.template ld<V>(vo, CV::soff(k)); },
+            [&](int k, V x) {
+                const V d = bq.template ld<V>(vo, CV::soff(k));
+                bq.st(V{a.one_m_fr * d.x + x.x, a.one_m_fr * d.y + x.y}, vo, CV::soff(k));
+            },
+            v, lds);
     }
     drain_memory();  // re-reads bq and gu
     // V4: C_p = sum_q Gr[q][p] B_q; gu += C_p d(Re u_k conj u_l)/du
@@ -657,15 +779,14 @@ __device__ __forceinline__ void mm_vjp(const MmCtx<R, N, T, M>& c, double zs, R 
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
         const Fld gu = c.fld(TR::GU + m), gx = c.fld(TR::GX + m);
-#pragma unroll
-        for (int k = 0; k < P; ++k) v[k] = gu.template ld<V>(vo, CV::soff(k));
-        block_fft<R, N, T, +1, TR::PLANAR>(v, c.tw, lds);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            R cs, sn;
-            cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
-            gx.st(V{cs * v[k].x + sn * v[k].y, cs * v[k].y - sn * v[k].x}, vo, CV::soff(k));
-        }
+        mm_xform<R, N, T, M, +1>(
+            c, [&](int k) { return gu.template ld<V>(vo, CV::soff(k)); },
+            [&](int k, V w) {
+                R cs, sn;
+                cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zs, cs, sn);
+                gx.st(V{cs * w.x + sn * w.y, cs * w.y - sn * w.x}, vo, CV::soff(k));
+            },
+            v, lds);
     }
 }
 
@@ -706,10 +827,10 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     V* slice = a.work + sol * (size_t)TR::FIELDS_GRAD * N;
     const Buf sl(slice, (uint32_t)((size_t)TR::FIELDS_GRAD * N * sizeof(V)));
     const Buf disp(a.disp, M * N * 8), hn(a.hn, N * sizeof(V)), shn(a.shn, N * sizeof(R));
-    const Buf tw(a.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf tw(a.tw, mm_twiddle_entries<R, N, T>() * sizeof(V));
     const Buf in(a.u0 + sol * M * N, M * N * sizeof(V));
     const int Q = a.q, QC = (a.q + 1) / 2;
-    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, Q, QC};
+    const MmCtx<R, N, T, M> c{a, sl, disp, hn, shn, tw, Q, QC, TR::XA};
     const Buf ckpt(a.ckpt + sol * (size_t)a.nseg * M * N, (uint32_t)((size_t)a.nseg * M * N * sizeof(V)));
     // stage tape: one region of 4*every slots per kept segment plus the shared
     // region of the recomputed segments (when the plan recomputes any)
@@ -726,7 +847,7 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
     // the last ntaped segments of this solve keep the forward sweep's tape
     const int first_kept = nseg - min(a.ntaped, nseg);
     auto seg_base = [&](int sg) { return (uint32_t)((sg >= first_kept ? sg - first_kept : a.ntaped) * 4 * a.every); };
-    V v[P];
+    V v[TR::PV];
     int bad = 0;
     double etot = 0.0, eband = 0.0;
     if (a.phase & 1) {
@@ -734,7 +855,7 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
             const Fld y = c.fld(m), st = c.fld(M + m);
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                const V x = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+                const V x = in.ld<V>(CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
                 y.st(x, vo, CV::soff(k));
                 st.st(x, vo, CV::soff(k));
             }
@@ -776,7 +897,7 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
                 const V x = st.template ld<V>(vo, CV::soff(k));
                 const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
                 bad |= !(isfinite(o.x) && isfinite(o.y));
-                out.st(o, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+                out.st(o, CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
                 lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
                 const double e = (double)o.x * o.x + (double)o.y * o.y;
                 etot += e;
@@ -806,7 +927,7 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
         for (int k = 0; k < P; ++k) {
             V s;
             if (a.seed) {
-                s = sd.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+                s = sd.ld<V>(CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
             } else {
                 const V o = lam.template ld<V>(vo, CV::soff(k));
                 const R f = (R)(((a.mask[threadIdx.x + T * k] ? 1.0 : 0.0) - J) * scale);
@@ -879,8 +1000,8 @@ __global__ __launch_bounds__(T) void k_grad_multi(MmArgs<R> a)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             const V l = lam.template ld<V>(vo, CV::soff(k));
-            const V u = in.ld<V>(vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
-            l0.st(l, vo, (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            const V u = in.ld<V>(CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
+            l0.st(l, CV::voff(), (uint32_t)(m * N * sizeof(V)) + CV::soff(k));
             g[k] += 2.0 * ((double)l.y * (double)u.x - (double)l.x * (double)u.y);
             bad |= !(isfinite(l.x) && isfinite(l.y));
         }

@@ -4,15 +4,19 @@
 namespace fh {
 namespace tm {
 
+#if FH_MM_PART != 32
 hipError_t launch_fwd_m10(int nt, const MtArgs<double>& a, int batch, hipStream_t s)
 {
     return by_size<double, 10, false>(nt, a, batch, s);
 }
+#endif
 
+#if FH_MM_PART != 64
 hipError_t launch_fwd_m10(int nt, const MtArgs<float>& a, int batch, hipStream_t s)
 {
     return by_size<float, 10, false>(nt, a, batch, s);
 }
+#endif
 
 }  // namespace tm
 }  // namespace fh

@@ -62,7 +62,8 @@ namespace fh {
 bool tsit5_supported(int nt, int m, bool fp64)
 {
     if (m > 1) return multi_supported(nt, m, fp64);
-    return m == 1 && nt >= 256 && nt <= (fp64 ? 16384 : 32768) && (nt & (nt - 1)) == 0;
+    // FP64 at 32768 samples runs on the multimode kernels with split transforms
+    return m == 1 && nt >= 256 && nt <= 32768 && (nt & (nt - 1)) == 0;
 }
 
 int tsit5_fields(int nt, bool fp64) { return fp64 ? fields_by_size<double>(nt) : fields_by_size<float>(nt); }

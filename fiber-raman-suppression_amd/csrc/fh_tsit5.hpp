@@ -48,6 +48,28 @@ struct Tsit5Tab {
                                        {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
                                        {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
                                        {0.0, 1.5, -4.0, 2.5}};
+    // a[i][j] and c[i] at a workgroup-uniform runtime stage index, as a
+    // select over immediates: the FP64 multimode units (built with forced
+    // memory waits) must not address constant tables through PC-relative
+    // sequences, which those waits would split.
+    static __device__ __forceinline__ double a_at(int i, int j)
+    {
+        double r = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 7; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 6; ++jj)
+                if (ii == i && jj == j) r = a[ii][jj];
+        return r;
+    }
+    static __device__ __forceinline__ double c_at(int i)
+    {
+        double r = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 7; ++ii)
+            if (ii == i) r = c[ii];
+        return r;
+    }
     static __device__ __forceinline__ double dense(int i, double th)
     {
         return th * (r[i][0] + th * (r[i][1] + th * (r[i][2] + th * r[i][3])));

@@ -13,6 +13,7 @@ template <typename R> hipError_t dispatch(int nt, int m, bool grad, const fh::Mt
 {
     using namespace fh::tm;
     switch (m) {
+    case 1: return grad ? launch_grad_m1(nt, a, batch, s) : launch_fwd_m1(nt, a, batch, s);
     case 2: return grad ? launch_grad_m2(nt, a, batch, s) : launch_fwd_m2(nt, a, batch, s);
     case 3: return grad ? launch_grad_m3(nt, a, batch, s) : launch_fwd_m3(nt, a, batch, s);
     case 4: return grad ? launch_grad_m4(nt, a, batch, s) : launch_fwd_m4(nt, a, batch, s);
@@ -26,9 +27,17 @@ template <typename R> hipError_t dispatch(int nt, int m, bool grad, const fh::Mt
 
 namespace fh {
 
-int tsit5_multi_fields(int m) { return 10 * m + (m * m + 1) / 2; }
+// Slice fields of the adaptive multimode kernels (MtLayout, MtgLayout), plus
+// the scratch of split transforms after them.
+int tsit5_multi_fields(int nt, int m, bool fp64)
+{
+    return 10 * m + (m * m + 1) / 2 + (mm_split_rt(nt, fp64) ? 2 : 0);
+}
 
-int tsit5_multi_grad_fields(int m) { return 15 * m + 2 * ((m * m + 1) / 2); }
+int tsit5_multi_grad_fields(int nt, int m, bool fp64)
+{
+    return 15 * m + 2 * ((m * m + 1) / 2) + (mm_split_rt(nt, fp64) ? 4 : 0);
+}
 
 hipError_t launch_tsit5_multi(int nt, int m, bool grad, const MtArgs<double>& a, int batch, hipStream_t s)
 {

@@ -33,7 +33,7 @@ template <typename R> struct MtArgs {
 
 template <int M> struct MtLayout {
     static constexpr int Y = 0, YN = M, K0 = 2 * M, U = 9 * M, HQ = 10 * M;
-    static constexpr int FIELDS = 10 * M + (M * M + 1) / 2;
+    static constexpr int FIELDS = 10 * M + (M * M + 1) / 2;  // split transforms add MmTraits::XTRA after these
 };
 
 // Gradient slice: the forward phase uses MtLayout; the backward phase keeps
@@ -44,7 +44,7 @@ template <int M> struct MtLayout {
 template <typename R, int N, int T, int M> struct MtgLayout {
     using TR = MmTraits<R, N, T, M>;
     static constexpr int Y = 0, U = M, HQ = 2 * M, K0 = TR::FIELDS_GRAD;
-    static constexpr int FIELDS = TR::FIELDS_GRAD + 6 * M;
+    static constexpr int FIELDS = TR::FIELDS_GRAD + 6 * M;  // split transforms add TR::XTRA after these
     static_assert(HQ + TR::QCMAX <= TR::LAM, "backward scratch must stay below the adjoint fields");
     static_assert(FIELDS >= MtLayout<M>::FIELDS, "the forward phase shares the slice");
 };
@@ -61,9 +61,9 @@ template <typename R, int N, int T, int M> struct MtgLayout {
 // shared region the backward sweep recomputes into.  A rejected attempt is
 // overwritten by the next one; the seventh stage is stage 0 of step n + 1.
 template <typename R, int N, int T, int M, bool CKPT>
-__device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const Buf& ckpt, const Buf& tape_u,
-                          const Buf& tape_h, int* ys_out, long long* nacc_out, long long* nrej_out,
-                          long long* nfe_out, C2<R>* v, void* lds)
+__device__ __forceinline__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const Buf& ckpt,
+                                          const Buf& tape_u, const Buf& tape_h, int* ys_out, long long* nacc_out,
+                                          long long* nrej_out, long long* nfe_out, C2<R>* v, void* lds)
 {
     using V = C2<R>;
     using TR = MmTraits<R, N, T, M>;
@@ -119,7 +119,7 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
                 }
                 R cs, sn;
                 cis_arg(c.disp.template ld<double>(CD::voff(), (uint32_t)(m * N * 8) + CD::soff(k)), zj, cs, sn);
-                dst.st(V{val.x * cs - val.y * sn, val.x * sn + val.y * cs}, vo, CV::soff(k));
+                dst.st(V{val.x * cs - val.y * sn, val.x * sn + val.y * cs}, CV::voff(), CV::soff(k));
             }
         }
     };
@@ -218,7 +218,7 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
 #pragma unroll
                     for (int j = 0; j < 6; ++j) {
                         if (j < i) {
-                            const R aij = (R)TB::a[i][j];
+                            const R aij = (R)TB::a_at(i, j);
                             const V kj = c.fld(kslot(j) + m).template ld<V>(vo, CV::soff(k));
                             s = V{s.x + aij * kj.x, s.y + aij * kj.y};
                         }
@@ -228,7 +228,7 @@ __device__ int mt_forward(const MtArgs<R>& a, const MmCtx<R, N, T, M>& c, const 
                 }
             }
             const uint32_t ts = CKPT ? (i < 6 ? tape_slot(nacc, i) : tape_slot(nacc + 1, 0)) : 0u;
-            mm_rhs<R, N, T, M, CKPT>(c, z + TB::c[i] * h, syn, kslot(i), LY::U, LY::HQ, CKPT ? tape_u : none,
+            mm_rhs<R, N, T, M, CKPT>(c, z + TB::c_at(i) * h, syn, kslot(i), LY::U, LY::HQ, CKPT ? tape_u : none,
                                      CKPT ? tape_h : none, ts, v, lds);
         }
         nfe += 6;
@@ -304,21 +304,21 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
     __shared__ __align__(16) unsigned char lds[TR::LDS_BYTES];
     const size_t sol = blockIdx.x;
     const MmArgs<R>& ma = a.m;
-    V* slice = ma.work + sol * (size_t)LY::FIELDS * N;
-    const Buf sl(slice, (uint32_t)((size_t)LY::FIELDS * N * sizeof(V)));
+    V* slice = ma.work + sol * (size_t)(LY::FIELDS + TR::XTRA) * N;
+    const Buf sl(slice, (uint32_t)((size_t)(LY::FIELDS + TR::XTRA) * N * sizeof(V)));
     const Buf disp(ma.disp, M * N * 8), hn(ma.hn, N * sizeof(V)), shn(ma.shn, N * sizeof(R));
-    const Buf tw(ma.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf tw(ma.tw, mm_twiddle_entries<R, N, T>() * sizeof(V));
     const Buf in(ma.u0 + sol * M * N, M * N * sizeof(V));
-    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, ma.q, (ma.q + 1) / 2};
+    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, ma.q, (ma.q + 1) / 2, LY::FIELDS};
     const uint32_t vo = CV::voff();
     const uint32_t FB = (uint32_t)(N * sizeof(V));
     const double L = ma.lengths ? ma.lengths[sol] : ma.length;
-    V v[P];
+    V v[TR::PV];
 #pragma unroll 1
     for (int m = 0; m < M; ++m) {
         const Fld y = c.fld(LY::Y + m);
 #pragma unroll
-        for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
+        for (int k = 0; k < P; ++k) y.st(in.ld<V>(CV::voff(), (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
     }
     int ys;
     long long nacc, nrej, nfe;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(T) void k_tsit5_multi(MtArgs<R> a)
             const V x = y.template ld<V>(vo, CV::soff(k));
             const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
             if (!(isfinite(o.x) && isfinite(o.y))) bad |= 1;
-            out.st(o, vo, (uint32_t)m * FB + CV::soff(k));
+            out.st(o, CV::voff(), (uint32_t)m * FB + CV::soff(k));
         }
     }
     if (bad) atomicOr(ma.status + sol, bad);
@@ -372,13 +372,13 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     const size_t sol = blockIdx.x;
     const MmArgs<R>& ma = a.m;
     const uint32_t FB = (uint32_t)(N * sizeof(V));
-    V* slice = ma.work + sol * (size_t)LB::FIELDS * N;
-    const Buf sl(slice, (uint32_t)((size_t)LB::FIELDS * N * sizeof(V)));
+    V* slice = ma.work + sol * (size_t)(LB::FIELDS + TR::XTRA) * N;
+    const Buf sl(slice, (uint32_t)((size_t)(LB::FIELDS + TR::XTRA) * N * sizeof(V)));
     const Buf disp(ma.disp, M * N * 8), hn(ma.hn, N * sizeof(V)), shn(ma.shn, N * sizeof(R));
-    const Buf tw(ma.tw, FftPlan<N, T>::TW_SIZE * sizeof(V));
+    const Buf tw(ma.tw, mm_twiddle_entries<R, N, T>() * sizeof(V));
     const Buf in(ma.u0 + sol * M * N, M * N * sizeof(V));
     const int Q = ma.q, QC = (ma.q + 1) / 2;
-    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, Q, QC};
+    const MmCtx<R, N, T, M> c{ma, sl, disp, hn, shn, tw, Q, QC, LB::FIELDS};
     const Buf ckpt(ma.ckpt + sol * (size_t)ma.nseg * M * N, (uint32_t)((size_t)ma.nseg * M * N * sizeof(V)));
     // stage tape: one region of 6*every slots per kept segment (the first
     // ntaped) plus the shared region of the recomputed segments
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
     const Buf tape_h(ma.tape_h + sol * tslots * QC * N, (uint32_t)(tslots * QC * N * sizeof(V)));
     const uint32_t vo = CV::voff();
     const double L = ma.lengths ? ma.lengths[sol] : ma.length;
-    V v[P];
+    V v[TR::PV];
     int ys = LF::Y, bad = 0;
     long long nacc, nrej, nfe;
     if (ma.phase & 1) {
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
         for (int m = 0; m < M; ++m) {
             const Fld y = c.fld(LF::Y + m);
 #pragma unroll
-            for (int k = 0; k < P; ++k) y.st(in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
+            for (int k = 0; k < P; ++k) y.st(in.ld<V>(CV::voff(), (uint32_t)m * FB + CV::soff(k)), vo, CV::soff(k));
         }
         bad = mt_forward<R, N, T, M, true>(a, c, ckpt, tape_u, tape_h, &ys, &nacc, &nrej, &nfe, v, lds);
         if (threadIdx.x == 0) {
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
                 const V x = y.template ld<V>(vo, CV::soff(k));
                 const V o = V{x.x * cs - x.y * sn, x.x * sn + x.y * cs};
                 bad |= !(isfinite(o.x) && isfinite(o.y));
-                out.st(o, vo, (uint32_t)m * FB + CV::soff(k));
+                out.st(o, CV::voff(), (uint32_t)m * FB + CV::soff(k));
                 lam.st(o, vo, CV::soff(k));  // lab-frame output, scaled into the seed below
                 const double e = (double)o.x * o.x + (double)o.y * o.y;
                 etot += e;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
         for (int k = 0; k < P; ++k) {
             V s;
             if (ma.seed) {
-                s = sd.ld<V>(vo, (uint32_t)m * FB + CV::soff(k));
+                s = sd.ld<V>(CV::voff(), (uint32_t)m * FB + CV::soff(k));
             } else {
                 const V o = lam.template ld<V>(vo, CV::soff(k));
                 const R f = (R)(((ma.mask[threadIdx.x + T * k] ? 1.0 : 0.0) - J) * scale);
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
 #pragma unroll
                             for (int j = 0; j < 5; ++j) {
                                 if (j < i) {
-                                    const R aij = (R)TB::a[i][j];
+                                    const R aij = (R)TB::a_at(i, j);
                                     const V kj = c.fld(LB::K0 + j * M + m).template ld<V>(vo, CV::soff(k));
                                     acc = V{acc.x + aij * kj.x, acc.y + aij * kj.y};
                                 }
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
                         }
                     }
                 }
-                mm_rhs<R, N, T, M, true>(c, z + TB::c[i] * h, i == 0 ? LB::Y : TR::GX, LB::K0 + i * M, LB::U,
+                mm_rhs<R, N, T, M, true>(c, z + TB::c_at(i) * h, i == 0 ? LB::Y : TR::GX, LB::K0 + i * M, LB::U,
                                          LB::HQ, tape_u, tape_h, rbase + (uint32_t)(6 * (s - s0) + i), v, lds);
             }
             if (s + 1 < s1) {
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
 #pragma unroll
                             for (int r = 1; r < 6; ++r) {
                                 if (r > i) {
-                                    const R ari = hr * (R)TB::a[r][i];
+                                    const R ari = hr * (R)TB::a_at(r, i);
                                     const V gy = c.fld(LB::K0 + r * M + m).template ld<V>(vo, CV::soff(k));
                                     g = V{g.x + ari * gy.x, g.y + ari * gy.y};
                                 }
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
                         }
                     }
                 }
-                mm_vjp<R, N, T, M>(c, z + TB::c[i] * h, hr * (R)TB::a[6][i], i < 5 ? (R)1 : (R)0, tape_u, tape_h,
+                mm_vjp<R, N, T, M>(c, z + TB::c_at(i) * h, hr * (R)TB::a_at(6, i), i < 5 ? (R)1 : (R)0, tape_u, tape_h,
                                    rbase + (uint32_t)(6 * (s - s0) + i), v, lds);
                 drain_memory();
 #pragma unroll 1
@@ -602,8 +602,8 @@ __global__ __launch_bounds__(T) void k_grad_tsit5_multi(MtArgs<R> a)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             const V l = lam.template ld<V>(vo, CV::soff(k));
-            const V u = in.ld<V>(vo, (uint32_t)m * FB + CV::soff(k));
-            l0.st(l, vo, (uint32_t)m * FB + CV::soff(k));
+            const V u = in.ld<V>(CV::voff(), (uint32_t)m * FB + CV::soff(k));
+            l0.st(l, CV::voff(), (uint32_t)m * FB + CV::soff(k));
             g[k] += 2.0 * ((double)l.y * (double)u.x - (double)l.x * (double)u.y);
             bad |= !(isfinite(l.x) && isfinite(l.y));
         }

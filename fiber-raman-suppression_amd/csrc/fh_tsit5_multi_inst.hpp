@@ -12,8 +12,10 @@ namespace tm {
 
 template <typename R, int N, int M, bool GRAD> hipError_t go(const MtArgs<R>& a, int batch, hipStream_t s)
 {
-    if constexpr (sizeof(R) == 8 && M == 2) {
-        return hipErrorInvalidValue;  // not offered (multi_supported)
+    if constexpr (M == 1 && !mm_split<R, N>()) {
+        // one mode runs here only where the single-mode kernels have no
+        // shape (split transforms)
+        return hipErrorInvalidValue;
     } else {
         constexpr int T = MultiShape<R, N>::T;
         if constexpr (GRAD)
@@ -34,14 +36,16 @@ template <typename R, int M, bool GRAD> hipError_t by_size(int nt, const MtArgs<
     case 4096: return go<R, 4096, M, GRAD>(a, batch, s);
     case 8192: return go<R, 8192, M, GRAD>(a, batch, s);
     case 16384: return go<R, 16384, M, GRAD>(a, batch, s);
-    case 32768:
-        if constexpr (sizeof(R) == 4) return go<R, 32768, M, GRAD>(a, batch, s);
-        return hipErrorInvalidValue;
+    case 32768: return go<R, 32768, M, GRAD>(a, batch, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 // Launchers, one pair per unit fh_tm_<fwd|grad>_m<M>.hip.
+hipError_t launch_fwd_m1(int nt, const MtArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_fwd_m1(int nt, const MtArgs<float>& a, int batch, hipStream_t s);
+hipError_t launch_grad_m1(int nt, const MtArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_grad_m1(int nt, const MtArgs<float>& a, int batch, hipStream_t s);
 hipError_t launch_fwd_m2(int nt, const MtArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_fwd_m2(int nt, const MtArgs<float>& a, int batch, hipStream_t s);
 hipError_t launch_grad_m2(int nt, const MtArgs<double>& a, int batch, hipStream_t s);

@@ -67,6 +67,8 @@ struct fh_plan_s {
     fh_tape spare = nullptr;         // storage of a freed tape, reused by the next fh_forward_tape
     int64_t gradient_bytes = 0;      // plan bytes once the adjoint buffers exist (layout)
     bool grad_work = false;          // multimode: the work slice has the gradient kernels' size
+    bool mm = false;     // runs on the multimode kernels (M > 1, or one mode with split transforms)
+    bool split = false;  // split transforms (FP64 at 32768 samples): Raman table index-reversed
 };
 
 // A kept forward solve (fh_forward_tape): the shaped launch, the output, the
@@ -255,9 +257,8 @@ int check_desc(const fh_problem_desc* d)
         if (!(std::isfinite(d->atol) && d->atol >= 0)) return fail(FH_EINVAL, "atol must be nonnegative and finite");
         if (d->max_steps < 0) return fail(FH_EINVAL, "max_steps must be >= 0");
         if (!fh::tsit5_supported(d->nt, d->modes, d->precision == FH_PREC_FP64))
-            return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..16384 samples "
-                                         "(FP32: 256..32768) and the multimode grids and mode counts of the "
-                                         "fixed-step scheme");
+            return fail(FH_EUNSUPPORTED, "the adaptive scheme covers single-mode grids of 256..32768 samples "
+                                         "and the multimode grids and mode counts of the fixed-step scheme");
     }
     if (d->max_batch <= 0) return fail(FH_EINVAL, "max_batch must be positive");
     if (!(std::isfinite(d->length_m) && d->length_m > 0)) return fail(FH_EINVAL, "fiber length must be positive and finite");
@@ -288,12 +289,9 @@ int check_desc(const fh_problem_desc* d)
     if (!(std::isfinite(d->objective_tau) && d->objective_tau >= 0))
         return fail(FH_EINVAL, "objective_tau must be finite and >= 0 (0 = 50)");
     if (d->modes == 1) {
-        const int nmax = d->precision == FH_PREC_FP64 ? 16384 : 32768;
-        if (d->nt < 64 || d->nt > nmax)
-            return fail(FH_EUNSUPPORTED, "nt must lie in [64, " + std::to_string(nmax) + "] for this precision");
+        if (d->nt < 64 || d->nt > 32768) return fail(FH_EUNSUPPORTED, "nt must lie in [64, 32768]");
     } else if (!fh::multi_supported(d->nt, d->modes, d->precision == FH_PREC_FP64)) {
-        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {3,4,6,10} with nt in [256, 16384] (FP64) and "
-                                     "M in {2,3,4,6,10} with nt in [256, 32768] (FP32)");
+        return fail(FH_EUNSUPPORTED, "multimode kernels cover M in {2,3,4,6,10} with nt in [256, 32768]");
     }
     return FH_OK;
 }
@@ -304,6 +302,8 @@ int check_desc(const fh_problem_desc* d)
 struct PlanLayout {
     int threads = 0, tw_entries = 0;
     bool adaptive = false;
+    bool mm = false;     // runs on the multimode kernels (M > 1, or one mode with split transforms)
+    bool split = false;  // FP64 fields of 32 samples per thread: transforms in two halves
     int every = 50, nseg = 0, ntaped = 0, max_steps = 0, q = 0;
     bool gamma_full_sym = true;
     std::vector<std::pair<int, int>> outp, inp;  // multimode output / input pairs (i, j)
@@ -320,8 +320,13 @@ size_t alloc_size(size_t bytes) { return bytes == 0 ? 16 : bytes; }  // dev_allo
 template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout& L)
 {
     const size_t nt = (size_t)d.nt, M = (size_t)d.modes, mb = (size_t)d.max_batch, V = 2 * sizeof(R);
-    const auto shape = with_size<R, OpShape>(d.nt, d.modes > 1);
+    const bool fp64 = sizeof(R) == 8;
+    L.split = fh::mm_split_rt(d.nt, fp64);
+    L.mm = M > 1 || L.split;
+    const auto shape = L.split ? std::make_pair(1024, fh::mm_twiddle_entries<double, 32768, 1024>())
+                               : with_size<R, OpShape>(d.nt, L.mm);
     if (shape.first == 0) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(d.nt));
+    if (L.split && d.nt != 32768) return fail(FH_EUNSUPPORTED, "no kernel for nt=" + std::to_string(d.nt));
     L.threads = shape.first;
     L.tw_entries = shape.second;
     L.adaptive = d.scheme == FH_SCHEME_TSIT5;
@@ -330,15 +335,15 @@ template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout&
         L.max_steps = d.max_steps > 0 ? d.max_steps : 10000;
         if (L.every > L.max_steps) L.every = L.max_steps;
         L.nseg = (L.max_steps + L.every - 1) / L.every;
-        L.work_per_solve = M == 1 ? (size_t)fh::tsit5_fields(d.nt, sizeof(R) == 8) * nt
-                                  : (size_t)fh::tsit5_multi_fields(d.modes) * nt;
+        L.work_per_solve = !L.mm ? (size_t)fh::tsit5_fields(d.nt, fp64) * nt
+                                 : (size_t)fh::tsit5_multi_fields(d.nt, d.modes, fp64) * nt;
     } else {
         if (L.every > d.steps) L.every = d.steps;
         L.nseg = (d.steps + L.every - 1) / L.every;
-        L.work_per_solve = M == 1 ? with_size<R, OpWork>(d.nt) : (size_t)fh::multi_fields(d.modes) * nt;
+        L.work_per_solve = !L.mm ? with_size<R, OpWork>(d.nt) : (size_t)fh::multi_fields(d.nt, d.modes, fp64) * nt;
     }
     L.ntaped = std::min(d.tape_segments, L.nseg);
-    if (M > 1) {
+    if (L.mm) {
         const int m = d.modes;
         auto G = [&](int i, int j, int k, int l) { return d.gamma[i + m * (j + m * (k + m * l))]; };
         double gmax = 0;
@@ -380,7 +385,7 @@ template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout&
     size_t fwd = alloc_size(8 * nt * M) + alloc_size(sizeof(R) * 2 * nt) + alloc_size(sizeof(R) * nt) +
                  alloc_size(sizeof(R) * 2 * (size_t)L.tw_entries);
     if (d.band_mask) fwd += alloc_size(nt);
-    if (M > 1) fwd += alloc_size(sizeof(R) * L.outp.size() * L.inp.size()) + alloc_size(sizeof(int) * M * M);
+    if (L.mm) fwd += alloc_size(sizeof(R) * L.outp.size() * L.inp.size()) + alloc_size(sizeof(int) * M * M);
     const size_t work = alloc_size(V * L.work_per_solve * mb);
     fwd += work + alloc_size(sizeof(int) * mb) + 2 * alloc_size(field);
     if (L.adaptive) fwd += 2 * alloc_size(sizeof(double) * mb * (size_t)L.max_steps) + alloc_size(sizeof(long long) * 3 * mb);
@@ -389,12 +394,12 @@ template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout&
     // shared region of recomputed segments: fixed step only when some
     // segments are recomputed; adaptive always, as the segment count is known
     // only at the end of a solve), adjoint fields
-    if (M > 1)
-        L.grad_work_per_solve =
-            (size_t)(L.adaptive ? fh::tsit5_multi_grad_fields(d.modes) : fh::multi_grad_fields(d.modes)) * nt;
+    if (L.mm)
+        L.grad_work_per_solve = (size_t)(L.adaptive ? fh::tsit5_multi_grad_fields(d.nt, d.modes, fp64)
+                                                    : fh::multi_grad_fields(d.nt, d.modes, fp64)) * nt;
     L.ckpt_per_solve = V * nt * M * (size_t)L.nseg;
     const size_t stages = L.adaptive ? 6 : 4;
-    const size_t hbytes = M == 1 ? sizeof(R) * nt : V * nt * (size_t)((L.q + 1) / 2);
+    const size_t hbytes = !L.mm ? sizeof(R) * nt : V * nt * (size_t)((L.q + 1) / 2);
     const size_t regions = (size_t)L.ntaped + ((L.adaptive || L.ntaped < L.nseg) ? 1 : 0);
     L.tape_u_per_solve = V * nt * M * stages * (size_t)L.every * regions;
     L.tape_h_per_solve = hbytes * stages * (size_t)L.every * regions;
@@ -403,10 +408,10 @@ template <typename R> int compute_layout_t(const fh_problem_desc& d, PlanLayout&
                        " bytes); raise checkpoint_every or lower steps / max_steps";
     else if (L.tape_u_per_solve > 0xffffffffull || L.tape_h_per_solve > 0xffffffffull)
         L.grad_error = "the stage tape of one solve exceeds 4 GiB; lower tape_segments or checkpoint_every";
-    else if (M > 1 && V * L.grad_work_per_solve > 0xffffffffull)
+    else if (L.mm && V * L.grad_work_per_solve > 0xffffffffull)
         L.grad_error = "the gradient work slice of one solve exceeds 4 GiB";
     size_t grad = fwd;
-    if (M > 1) grad = grad - work + alloc_size(V * L.grad_work_per_solve * mb);
+    if (L.mm) grad = grad - work + alloc_size(V * L.grad_work_per_solve * mb);
     grad += alloc_size(L.ckpt_per_solve * mb) + alloc_size(L.tape_u_per_solve * mb) +
             alloc_size(L.tape_h_per_solve * mb) + 2 * alloc_size(field) + alloc_size(sizeof(double) * nt * mb) +
             alloc_size(sizeof(double) * mb);
@@ -424,20 +429,39 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
     const fh_problem_desc& d = p->d;
     const int nt = d.nt;
     std::vector<std::complex<long double>> tw;
-    const int T = d.modes == 1 ? with_size<R, OpTw>(nt, tw) : with_size<R, OpTwMulti>(nt, tw);
+    int T = 0;
+    if (!L.mm) {
+        T = with_size<R, OpTw>(nt, tw);
+    } else if (!L.split) {
+        T = with_size<R, OpTwMulti>(nt, tw);
+    } else {
+        // split transforms: the N/2-point block plan, then w^n = exp(-2 pi i n / N), n < N/2
+        T = 1024;
+        fill_twiddles<16384, 1024>(tw);
+        const long double two_pi = 6.283185307179586476925286766559005768L;
+        for (int n = 0; n < nt / 2; ++n) {
+            const long double a = -two_pi * (long double)n / (long double)nt;
+            tw.emplace_back(cosl(a), sinl(a));
+        }
+    }
     if (T != L.threads || (int)tw.size() != L.tw_entries)
         return fail(FH_EINVAL, "internal: kernel shape differs from the plan layout");
     p->threads = T;
+    p->mm = L.mm;
+    p->split = L.split;
     std::vector<R> twr(2 * tw.size()), hn(2 * (size_t)nt), shn(nt);
     for (size_t i = 0; i < tw.size(); ++i) {
         twr[2 * i] = (R)tw[i].real();
         twr[2 * i + 1] = (R)tw[i].imag();
     }
-    for (int k = 0; k < nt; ++k) {
+    // hR (-1)^k / N; split transforms convolve through the +1 transform
+    // first and take the table at index -k
+    for (int e = 0; e < nt; ++e) {
+        const int k = L.split ? (nt - e) % nt : e;
         const double sgn = (k & 1) ? -1.0 : 1.0;
-        hn[2 * k] = (R)(d.raman[2 * k] * sgn / nt);
-        hn[2 * k + 1] = (R)(d.raman[2 * k + 1] * sgn / nt);
-        shn[k] = (R)(d.shock[k] / nt);
+        hn[2 * e] = (R)(d.raman[2 * k] * sgn / nt);
+        hn[2 * e + 1] = (R)(d.raman[2 * k + 1] * sgn / nt);
+        shn[e] = (R)(d.shock[e] / nt);
     }
     int rc;
     if ((rc = dev_alloc(p, p->disp, sizeof(double) * (size_t)nt * d.modes))) return rc;
@@ -468,7 +492,7 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
     p->grad_error = L.grad_error;
     p->q = L.q;
     p->gamma_full_sym = L.gamma_full_sym;
-    if (d.modes > 1) {
+    if (L.mm) {
         const int M = d.modes;
         auto G = [&](int i, int j, int k, int l) { return d.gamma[i + M * (j + M * (k + M * l))]; };
         const bool sym = L.q != M * M;
@@ -544,7 +568,7 @@ template <typename R> fh::SmArgs<R> base_args(fh_plan p)
 template <typename R> int ensure_grad_work(fh_plan p)
 {
     if (!p->grad_error.empty()) return fail(FH_EINVAL, p->grad_error);
-    if (p->grad_work || p->d.modes == 1) return FH_OK;
+    if (p->grad_work || !p->mm) return FH_OK;
     total_release(p, p->work);
     int rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->grad_work_per_solve * (size_t)p->d.max_batch);
     if (rc) return rc;
@@ -612,13 +636,13 @@ int launch_gradient_with(fh_plan p, const GradBufs& gb, int batch, const void* d
 {
     if (!d_seed && !p->has_mask)
         return fail(FH_EINVAL, "gradient needs a terminal seed or a plan band mask");
-    if (p->d.modes > 1) {
+    if (p->mm) {
         if (!p->gamma_full_sym)
             return fail(FH_ENOTSYM, "fiber adjoint requires a fully permutation-symmetric gamma tensor");
         if (!p->adaptive && !d_lengths != !d_nsteps)
             return fail(FH_EINVAL, "multimode per-solve lengths need per-solve step counts");
     }
-    if (p->d.modes > 1) {
+    if (p->mm) {
         const fh_problem_desc& d = p->d;
         fh::MmArgs<R> m{};
         m.u0 = (const fh::C2<R>*)d_u0;
@@ -780,7 +804,7 @@ int launch_forward(fh_plan p, int batch, const void* d_u0, void* d_uL, hipStream
                    int nz = 0, void* d_saved = nullptr)
 {
     const fh_problem_desc& d = p->d;
-    if (d.modes > 1)
+    if (p->mm)
         return launch_forward_multi<R>(p, batch, d_u0, d_uL, s, d_lengths, d_nsteps, d_zsave, nz, d_saved);
     if (d_lengths && !p->adaptive) return fail(FH_EINVAL, "per-solve lengths need the adaptive scheme");
     fh::SmArgs<R> a = base_args<R>(p);
@@ -1058,7 +1082,7 @@ ScratchLayout scratch_layout(fh_plan p, int batch)
 {
     const size_t B = (size_t)batch, nt = (size_t)p->d.nt, M = (size_t)p->d.modes;
     const size_t V = p->d.precision == FH_PREC_FP64 ? 16 : 8;
-    const size_t work = M > 1 ? p->grad_work_per_solve : p->work_per_solve;
+    const size_t work = p->mm ? p->grad_work_per_solve : p->work_per_solve;
     const size_t steps = p->adaptive ? (size_t)p->max_steps : 0;
     ScratchLayout L{};
     size_t off = 0;
@@ -1367,6 +1391,7 @@ int fh_field_metrics(fh_plan p, int32_t count, const double* spectra, const doub
     const size_t nt = (size_t)p->d.nt, m = (size_t)p->d.modes;
     const bool fp64 = p->d.precision == FH_PREC_FP64;
     const size_t esz = fp64 ? 16 : 8;
+    if (p->split) return fail(FH_EUNSUPPORTED, "field metrics: no FP64 kernel at nt=" + std::to_string(nt));
     if (m * nt * esz >= ((size_t)1 << 31)) return fail(FH_EINVAL, "spectrum too large");
     for (size_t k = 0; k < nt; ++k)
         if (!(std::isfinite(omega[k]) && omega[k] > 0)) return fail(FH_EINVAL, "omega must be positive and finite");
@@ -1636,7 +1661,7 @@ int fh_adjoint(fh_tape t, const double* seed, double* lam0, double* grad_phase, 
     // a single-mode backward sweep overwrites the kept tape (each slot is
     // pulled back once): sweeping again rebuilds it from the kept launch,
     // which reproduces the same bits
-    if (t->swept && M == 1 && p->ntaped > 0) {
+    if (t->swept && !p->mm && p->ntaped > 0) {
         if ((rc = tape_forward(t, s))) return rc;
     }
     if (seed) {

@@ -243,14 +243,16 @@ class AdjointModel:
         """Kept segments whose regions (plus the shared recomputation region)
         fit in TAPE_BUDGET_BYTES: a stage of the tape holds u (complex) and
         h (real, single mode) or u of every mode and ceil(Q/2) complex h pairs
-        (multimode), 4 stages per step."""
+        (multimode kernels, which also serve one FP64 mode at 32768 samples),
+        4 stages per step."""
         esz = 8 if precision == "fp64" else 4
         nt, m = problem.nt, problem.modes
         # the plan's rule (fh_plan_create): checkpoint_every = 0 means 50,
         # and the spacing never exceeds the step count
         every = min(int(checkpoint_every) if int(checkpoint_every) > 0 else 50, int(steps))
         q = m * (m + 1) // 2
-        per_stage = nt * 3 * esz if m == 1 else nt * (m + (q + 1) // 2) * 2 * esz
+        multimode = m > 1 or (precision == "fp64" and nt > 16384)
+        per_stage = nt * (m + (q + 1) // 2) * 2 * esz if multimode else nt * 3 * esz
         per_segment = 4 * every * per_stage
         nseg = -(-int(steps) // every)
         return int(max(0, min(nseg, cls.TAPE_BUDGET_BYTES // per_segment - 1)))

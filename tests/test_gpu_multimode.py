@@ -53,11 +53,6 @@ def test_mode_counts_match_oracle(gpu_context, modes, nt, precision):
                                 length_m=0.002, energy_j=2e-9, nx=96, window_um=60.0)
     steps = 20
     print(f"case: modes={modes} nt={nt} {precision}", flush=True)
-    if modes == 2 and precision == "fp64":
-        # the two-mode FP64 kernel is not offered (reproducibility); the plan says so
-        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
-            fiberhip.Plan.from_problem(problem, steps=10, precision=precision)
-        return
     gpu, _ = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision).forward(problem.uω0)
     ref = _oracle(problem, steps)
     if precision == "fp64":

@@ -8,8 +8,9 @@ requests submitted together.
 * two forward requests solved again by the oracle at their (short) length:
   FP32-bulk within 1e-5 relative spectral energy per mode, and the field
   within 1e-4 (fields move mostly in phase);
-* the FP64 server serves the 16384-sample multimode class and rejects the
-  32768-sample one with FH_EUNSUPPORTED at submission.
+* the FP64 server serves the 16384- and 32768-sample multimode classes (the
+  latter on split transforms), each result equal to the FP64 plan API's bit
+  for bit.
 """
 from __future__ import annotations
 
@@ -100,17 +101,16 @@ def test_large_classes_match_plan_api_and_oracle(gpu_context):
         assert rel(got[i]["uL"], ref) <= 1e-4
 
 
-def test_fp64_server_covers_16384_multimode(gpu_context):
+def test_fp64_server_covers_large_multimode_classes(gpu_context):
     import fiberhip
 
-    reqs = [r for r in _requests() if r[0].nt == 16384][:2] + [r for r in _requests() if r[0].nt == 32768][:1]
-    print("case: serving fp64 16384 multimode", flush=True)
+    reqs = [r for r in _requests() if r[0].nt == 16384][:2] + [r for r in _requests() if r[0].nt == 32768][:2]
+    print("case: serving fp64 16384 and 32768 multimode", flush=True)
     with fiberhip.Server(lanes=2, max_batch=2, precision="fp64", multimode_step_m=STEP_M,
                          mm_grad_max_steps=64) as srv:
-        got = [srv.wait(_submit(srv, r)) for r in reqs[:2]]
-        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
-            _submit(srv, reqs[2])
-    for req, res in zip(reqs[:2], got):
+        ids = [_submit(srv, r) for r in reqs]
+        got = [srv.wait(i) for i in ids]
+    for req, res in zip(reqs, got):
         p, kind, length, u0 = req
         steps = max(16, int(np.ceil(length / STEP_M)))
         fib = dict(p.fiber)

@@ -258,6 +258,30 @@ def test_plan_check_rejects_gradients_beyond_32bit_regions():
     assert grad - fwd == 4 * (ckpt + tape + 2 * 16384 * 16 + 16384 * 8 + 8)
 
 
+def test_plan_check_fp64_32768_single_mode_on_split_kernels():
+    """One FP64 mode at 32768 samples runs on the multimode kernels with split
+    transforms: its layout has their slice (7 fields forward, 13 for the
+    gradient, two of them the split scratch) and a complex h tape per stage;
+    five modes and 65536 samples have no kernel."""
+    from fiberhip.engine import Plan
+
+    nt = 32768
+    args, kw = _check_args(nt, steps=100, checkpoint_every=50)
+    fwd, grad = Plan.check(*args, **kw), Plan.check(*args, gradient=True, **kw)
+    args0, kw0 = _check_args(nt, steps=100, checkpoint_every=50, band=False)
+    assert fwd - Plan.check(*args0, **kw0) == nt
+    slice_growth = (13 - 7) * 16 * nt
+    ckpt = 2 * 16 * nt
+    tape = 2 * (4 * 50 * 16 * nt)  # one shared region of u and of h
+    assert grad - fwd == slice_growth + ckpt + tape + 2 * 16 * nt + 8 * nt + 8
+    args, kw = _check_args(nt, steps=None, scheme="tsit5", max_steps=200)
+    assert Plan.check(*args, gradient=True, **kw) > 0
+    for bad_nt, bad_m in ((nt, 5), (65536, 1)):
+        args, kw = _check_args(bad_nt, modes=bad_m, steps=10)
+        with pytest.raises(fiberhip.FiberHIPError, match="FH_EUNSUPPORTED"):
+            Plan.check(*args, **kw)
+
+
 def test_plan_check_validates_like_plan_create():
     """The descriptor checks of fh_plan_create run on the host: unknown
     objectives, a negative temperature and a zero step count are rejected."""
