@@ -25,6 +25,15 @@ sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 import numpy as np  # noqa: E402
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense vector/matrix peaks
+# Matched error of the headline metric: relative output spectral energy of the
+# FP32-bulk solve against the FP64 solve of the same launch.
+MATCHED_ENERGY_TOL = 1e-5
+# Normwise bound on the FP32-bulk phase gradient against FP64 that the GPU tests
+# assert on this workload at full size (tests/test_gpu_smf28_2m_gradient.py).
+GRADIENT_NORMWISE_BOUND = 5e-4
+# The GPU box's CPU share for one GPU: the all-core CPU baseline uses at most
+# this many threads, whatever the affinity mask shows.
+HOST_CORE_SHARE = 16
 # HBM bytes per unit of work measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled x2.0 / x1.0 by a plain copy kernel of known size); the summaries and
 # commands are in profiles/.
@@ -82,13 +91,15 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def launches(problem, batch: int, seed: int = 3) -> np.ndarray:
+def launches(problem, batch: int, seed: int = 3, orders=(2, 3, 4)) -> np.ndarray:
+    """Launch spectra with seeded Taylor phases: GDD ~ U(-1e4, 1e4) fs^2,
+    TOD ~ U(-5e5, 5e5) fs^3 and, when order 4 is asked for, FOD ~ U(-1e7, 1e7) fs^4."""
     import fiberhip
 
     rng = np.random.default_rng(seed)
-    basis = fiberhip.taylor_phase_basis(problem, (2, 3, 4))
-    coef = np.stack([rng.uniform(-1e4, 1e4, batch), rng.uniform(-5e5, 5e5, batch),
-                     rng.uniform(-1e7, 1e7, batch)], axis=1)
+    basis = fiberhip.taylor_phase_basis(problem, orders)
+    span = {2: 1e4, 3: 5e5, 4: 1e7}
+    coef = np.stack([rng.uniform(-span[o], span[o], batch) for o in orders], axis=1)
     phases = coef @ basis.T  # (batch, nt)
     return problem.uω0[None, :, 0] * np.exp(1j * phases)  # (batch, nt)
 
@@ -139,7 +150,7 @@ def time_gradient(problem, steps, precision, batch, k, warmup, mask, tape_segmen
 
     plan = fiberhip.Plan.from_problem(problem, steps=steps, precision=precision, max_batch=batch, band_mask=mask,
                                       checkpoint_every=50, tape_segments=tape_segments)
-    u0 = plan.to_device(launches(problem, batch, seed=2)[:, :, None])
+    u0 = plan.to_device(launches(problem, batch, seed=2, orders=(2, 3))[:, :, None])
     out = fiberhip.DeviceArray(plan.context, u0.nbytes)
     lam0 = fiberhip.DeviceArray(plan.context, u0.nbytes)
     grad = fiberhip.DeviceArray(plan.context, 8 * batch * problem.nt)
@@ -149,9 +160,13 @@ def time_gradient(problem, steps, precision, batch, k, warmup, mask, tape_segmen
     print(f"bench: {precision} gradient warmup done, timing {k} launches of {batch}", file=sys.stderr, flush=True)
     ms = plan.time_gradient_device(batch, u0.ptr, out.ptr, lam0.ptr, grad.ptr, cost.ptr, k)
     energies = plan.energy_device(batch, out.ptr)
-    return dict(kernel_ms=float(np.mean(ms)), ms=ms, energies=energies,
-                grad=grad.to_host(np.float64, (batch, problem.nt)), cost=cost.to_host(np.float64, (batch,)),
-                device_bytes=plan.device_bytes)
+    res = dict(kernel_ms=float(np.mean(ms)), ms=ms, energies=energies,
+               grad=grad.to_host(np.float64, (batch, problem.nt)), cost=cost.to_host(np.float64, (batch,)),
+               device_bytes=plan.device_bytes)
+    for a in (u0, out, lam0, grad, cost):
+        a.free()
+    plan.close()
+    return res
 
 
 def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol=1e-6, atol=1e-6, tape_segments=0):
@@ -270,78 +285,120 @@ def run_serving(trace, precision, rate, cache, lanes=8, max_batch=32):
     return results, rejected, wall
 
 
-def cpu_baseline_gradient(problem, steps, mask, slice_steps=200):
-    """The oracle's C discrete adjoint (FP64, one thread per gradient) on the
-    host cores.  Bounded sample: every thread runs one gradient over the first
-    `slice_steps` of the workload's RK4 steps (same grid, step size and
-    checkpoint spacing); the per-step cost is uniform, so the full-length
-    rate is the sample rate times slice_steps/steps."""
+def _host_description() -> dict:
+    """CPU model, core counts and the oracle's compiler flags, for the baseline's record."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    flags = ""
+    try:
+        for ln in (ROOT / "oracle" / "Makefile").read_text().splitlines():
+            if ln.startswith("CFLAGS"):
+                flags = ln.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    import subprocess
+
+    try:
+        cc = "gcc " + subprocess.run(["gcc", "-dumpfullversion"], capture_output=True, text=True,
+                                     timeout=10).stdout.strip()
+    except (OSError, subprocess.SubprocessError):
+        cc = "gcc"
+    return dict(cpu_model=model, nproc=os.cpu_count(), affinity_cores=len(os.sched_getaffinity(0)),
+                compiler=cc, cflags=flags)
+
+
+def _median_of_runs(fn, runs: int = 5):
+    """One untimed warm-up, then the median wall time of `runs` timed calls."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts
+
+
+def _parallel(threads: int, work) -> None:
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+
+
+def cpu_baseline_gradient(problem, steps, mask, slice_steps=50, runs=5):
+    """The oracle's C discrete adjoint (FP64, one thread per gradient, as the
+    reference pins one FFT thread per solve) on the host cores.  Bounded
+    sample: each gradient runs over the first `slice_steps` of the workload's
+    RK4 steps (same grid, step size and checkpoint spacing); the per-step cost
+    is uniform, so full-length figures are the sample's scaled by
+    steps/slice_steps.  Protocol: one warm-up, median of `runs` timed runs, for
+    (i) one gradient on one thread and (ii) one gradient per thread on every
+    core of the GPU's host share in parallel."""
     import fiberhip
     import oracle
     from oracle import np_ref
 
     f = problem.fiber
     fib = oracle.Fiber(f["Dω"], f["γ"], f["hRω"], fiberhip.self_steepening(problem.sim), f["one_m_fR"])
-    u = launches(problem, 16, seed=2)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = max(1, min(HOST_CORE_SHARE, len(os.sched_getaffinity(0))))
+    u = launches(problem, threads, seed=2, orders=(2, 3))
     h = f["L"] / steps
     length = h * slice_steps
 
     def work(i):
-        u0 = u[i % 16][:, None]
+        u0 = u[i % threads][:, None]
         uL = oracle.rk4_forward(fib, length, slice_steps, u0)
         _, seed = np_ref.band_cost(uL, mask)
         oracle.rk4_adjoint(fib, length, slice_steps, u0, seed, every=50)
 
-    t = time.perf_counter()
-    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
-    for th in ths:
-        th.start()
-    for th in ths:
-        th.join()
-    wall = time.perf_counter() - t
-    value = threads / wall * slice_steps / steps
+    single, _ = _median_of_runs(lambda: work(0), runs)
+    allcore, _ = _median_of_runs(lambda: _parallel(threads, work), runs)
+    scale = steps / slice_steps
+    value = threads / (allcore * scale)
     return dict(value=value, unit="grads/s", cores=threads, kind="port",
-                sample=f"{threads} SMF-28 2 m gradients (forward + discrete adjoint, checkpoint every 50) over the first "
-                       f"{slice_steps} of {steps} RK4 steps at Nt={problem.nt}, one per thread, {wall:.1f} s wall; "
-                       f"rate scaled by {slice_steps}/{steps}")
+                single_core_latency_s=single * scale, all_core_grads_per_s=value,
+                protocol=f"one warm-up, median of {runs} timed runs; sample of {slice_steps} of {steps} RK4 steps "
+                         f"per gradient, scaled by {steps}/{slice_steps}",
+                sample=f"SMF-28 2 m gradients (forward + discrete adjoint, checkpoint every 50) over the first "
+                       f"{slice_steps} of {steps} RK4 steps at Nt={problem.nt}: {threads} in parallel, one per "
+                       f"thread (median {allcore:.2f} s), and one alone on one thread (median {single:.2f} s); "
+                       f"rates scaled by {slice_steps}/{steps}",
+                **_host_description())
 
 
-def cpu_baseline(problem, steps, budget_s=20.0):
-    """The oracle's C restatement (FP64, one thread per solve) on the host
-    cores: all-core throughput of independent solves, as the reference
-    recommends parallelising solves rather than FFTs."""
+def cpu_baseline(problem, steps, runs=5):
+    """The oracle's C restatement of the fixed-step forward (FP64, one thread
+    per solve) on the host cores: one warm-up, then the median of `runs` timed
+    runs for (i) one solve on one thread and (ii) one solve per thread on every
+    core of the GPU's host share in parallel (the reference parallelises
+    independent solves, not transforms)."""
     import fiberhip
     import oracle
 
     f = problem.fiber
     fib = oracle.Fiber(f["Dω"], f["γ"], f["hRω"], fiberhip.self_steepening(problem.sim), f["one_m_fR"])
-    u = launches(problem, 64)
-    t = time.perf_counter()
-    oracle.rk4_forward(fib, f["L"], steps, u[0][:, None])
-    single = time.perf_counter() - t
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    per_thread = max(1, int(budget_s / single / 2))
-    done = [0]
-    lock = threading.Lock()
+    threads = max(1, min(HOST_CORE_SHARE, len(os.sched_getaffinity(0))))
+    u = launches(problem, threads)
 
     def work(i):
-        for j in range(per_thread):
-            oracle.rk4_forward(fib, f["L"], steps, u[(i * per_thread + j) % 64][:, None])
-            with lock:
-                done[0] += 1
+        oracle.rk4_forward(fib, f["L"], steps, u[i % threads][:, None])
 
-    t = time.perf_counter()
-    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
-    for th in ths:
-        th.start()
-    for th in ths:
-        th.join()
-    wall = time.perf_counter() - t
-    return dict(value=done[0] / wall, unit="solves/s", cores=threads, kind="port",
-                sample=f"{done[0]} SMF-28 0.5 m solves ({steps} RK4 steps, Nt={problem.nt}) on {threads} threads, "
-                       f"one solve per thread at a time; single-thread latency {single:.3f} s",
-                single_thread_s=single)
+    single, _ = _median_of_runs(lambda: work(0), runs)
+    allcore, _ = _median_of_runs(lambda: _parallel(threads, work), runs)
+    return dict(value=threads / allcore, unit="solves/s", cores=threads, kind="port",
+                single_core_latency_s=single, all_core_solves_per_s=threads / allcore,
+                protocol=f"one warm-up, median of {runs} timed runs",
+                sample=f"SMF-28 0.5 m solves ({steps} RK4 steps, Nt={problem.nt}): {threads} in parallel, one per "
+                       f"thread (median {allcore:.3f} s), and one alone on one thread (median {single:.3f} s)",
+                **_host_description())
 
 
 def main() -> None:
@@ -400,8 +457,22 @@ def main() -> None:
         achieved = args.batch * flops / (ms / 1e3) / 1e12
         e32, e64 = g["fp32"]["energies"], g["fp64"]["energies"]
         gerr = np.linalg.norm(g["fp32"]["grad"] - g["fp64"]["grad"], axis=1) / np.linalg.norm(g["fp64"]["grad"], axis=1)
+        # the matched-error condition of the metric: output spectral energy of every
+        # FP32 solve within 1e-5 of the FP64 solve of the same launch
+        energy_err = float(np.max(np.abs(e32 - e64) / e64))
+        matched = energy_err <= MATCHED_ENERGY_TOL
+        metric = "adjoint grads/sec (FP32-bulk batched path) vs plain FP64 GPU @<=1e-5 rel spectral-energy err"
+        if not matched:
+            metric += f" (tolerance NOT met: {energy_err:.2e})"
+        # batch-1 latency: one gradient alone on the GPU (the L-BFGS loop's use), every segment's tape kept
+        lat = {}
+        for p in ("fp32", "fp64"):
+            r1 = time_gradient(problem, steps, p, 1, 2, 1, mask, 40)
+            lat[f"{p}_ms"] = r1["kernel_ms"]
+        print(f"bench: batch-1 gradient {lat['fp32_ms']:.1f} ms fp32, {lat['fp64_ms']:.1f} ms fp64",
+              file=sys.stderr, flush=True)
         line = {
-            "metric": "adjoint grads/sec (FP32-bulk batched path) vs plain FP64 GPU @<=1e-5 err",
+            "metric": metric,
             "value": value,
             "unit": "grads/s",
             "n_gpus": 1,
@@ -420,12 +491,17 @@ def main() -> None:
                        "kept_tape_segments": kept, "tape_budget_gib": args.tape_budget_gib},
             "fp64_value": fp64_value,
             "fp32_over_fp64": value / fp64_value,
-            "fp32_vs_fp64": {"max_rel_output_spectral_energy_err": float(np.max(np.abs(e32 - e64) / e64)),
+            "fp32_vs_fp64": {"max_rel_output_spectral_energy_err": energy_err,
+                             "tolerance": MATCHED_ENERGY_TOL, "within_tolerance": matched,
                              "max_rel_cost_err": float(np.max(np.abs(g["fp32"]["cost"] - g["fp64"]["cost"])
                                                               / np.abs(g["fp64"]["cost"]))),
-                             "max_rel_gradient_err_normwise": float(np.max(gerr))},
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                             "max_rel_gradient_err_normwise": float(np.max(gerr)),
+                             "gradient_normwise_bound": GRADIENT_NORMWISE_BOUND},
+            "batch1_gradient_latency_ms": lat,
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                          "frac": achieved / PEAK_TFLOPS["fp32"],
+                         "limiter": "vector ALU and LDS: the block transforms' LDS exchanges run at 86% of the "
+                                    "measured LDS throughput; no matrix-core instructions in this kernel",
                          "traffic": MEASURED_HBM_BYTES["k_grad_single<float,16384,1024>"][0] * args.batch,
                          "traffic_source": MEASURED_HBM_BYTES["k_grad_single<float,16384,1024>"][1],
                          "kernel": "k_grad_single<float,16384,1024>", "flops_per_gradient": flops},
@@ -435,8 +511,7 @@ def main() -> None:
             "device_bytes_fp64_plan": g["fp64"]["device_bytes"],
         }
         if not args.no_cpu:
-            cb = cpu_baseline_gradient(problem, steps, mask)
-            line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            line["cpu_baseline"] = cpu_baseline_gradient(problem, steps, mask)
     if args.only in ("all", "forward"):
         problem, steps = workload("smf28-fwd-8192")
         nt, m = problem.nt, problem.modes
@@ -454,9 +529,11 @@ def main() -> None:
             "fp32_roofline_frac": args.fwd_batch * flops / (res["fp32"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp32"],
             "fp64_roofline_frac": args.fwd_batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp64"],
         }
-        if not args.no_cpu and args.only == "forward":
-            cb = cpu_baseline(problem, steps)
-            fwd["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        # batch-1 latency: one solve alone on the GPU
+        fwd["batch1_latency_ms"] = {f"{p}_ms": time_path(problem, steps, p, 1, 5, 1)["kernel_ms"]
+                                    for p in ("fp32", "fp64")}
+        if not args.no_cpu:
+            fwd["cpu_baseline"] = cpu_baseline(problem, steps)
         if args.only == "forward":
             line = {"metric": "fwd solves/sec (FP32-bulk batched path)", "value": fwd["fp32_solves_per_s"],
                     "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
