@@ -38,6 +38,12 @@ int tsit5_fields(int nt, bool fp64);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<double>& a, int batch, hipStream_t s);
 hipError_t launch_tsit5(int nt, bool grad, const TsArgs<float>& a, int batch, hipStream_t s);
 
+// Pump + signal (gain) solves (fh_gain.hip): single mode, 256..16384 samples.
+bool gain_supported(int nt);
+int gain_fields(int nt, bool fp64);
+hipError_t launch_tsit5_gain(int nt, const TsArgs<double>& a, int batch, hipStream_t s);
+hipError_t launch_tsit5_gain(int nt, const TsArgs<float>& a, int batch, hipStream_t s);
+
 // Terminal objectives and physical gradients on device fields (fh_objective.hip).
 constexpr int kObjectiveMaxModes = 16;
 hipError_t launch_objective(bool fp64, const void* uL, const unsigned char* mask, int nt, int modes, int kind,

@@ -76,6 +76,19 @@ struct Tsit5Tab {
     }
 };
 
+// Gain model of the pump + signal solve (disp_gain_smf!,
+// simulate_disp_gain_mmf.jl:21-63, with calculate_gain_YDFA, gain.jl:148-167)
+// as coefficients on |x_k|^2 in FFT order: coef rows w12, w21, ga, ge of N
+// doubles each; W12 = sum w12 |x|^2, W21 = sum w21 |x|^2, R12 = r12 Pp,
+// R21 = r21 Pp, n2 = (R12 + W12) / (R12 + R21 + W12 + W21 + inv_tau),
+// g_k = ge_k n2 - ga_k n1, gP = gpe n2 - gpa n1.
+struct GainDev {
+    const double* coef = nullptr;
+    double r12 = 0, r21 = 0, inv_tau = 0, gpa = 0, gpe = 0;
+    const double* pump0 = nullptr;  // [batch] Pp(0)
+    double* pumpL = nullptr;        // [batch] Pp(L)
+};
+
 template <typename R> struct TsArgs {
     SmArgs<R> s;
     double rtol, atol;
@@ -88,6 +101,7 @@ template <typename R> struct TsArgs {
     const double* zsave;  // [nz] sorted save positions in [0, L], shared by the batch
     int nz;               // 0: no dense output
     C2<R>* saved;         // [batch][nz][N] lab-frame spectra cis(D z) y(z)
+    GainDev gain;         // pump + signal solves only
     __device__ __forceinline__ double length(size_t sol) const { return lengths ? lengths[sol] : s.length; }
 };
 
@@ -141,6 +155,66 @@ template <typename R, int N, int T> struct TsLayout {
     static constexpr int NW_FWD = CORE_FWD + EXTRA, NW_GRAD = CORE_GRAD + EXTRA;
 };
 
+// Gain terms of disp_gain_smf! at one stage evaluation (before the
+// right-hand side): the population n2 from the stage input x (W12, W21 summed
+// in FP64 over the workgroup) and the stage pump pin; x is kept in slot gs
+// for the term g/2 .* x that gain_post adds to the right-hand side.  Returns
+// n2; *kp = gP pin, the pump's derivative.
+template <typename R, int N, int T>
+__device__ __forceinline__ double gain_pre(const GainDev& g, const C2<R>* v, const Fld& gs, double pin, double* kp,
+                                           void* lds)
+{
+    using V = C2<R>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = N / T;
+    const Buf cf(g.coef, (uint32_t)(4 * N * sizeof(double)));
+    constexpr uint32_t ROW = N * sizeof(double);
+    double s12 = 0, s21 = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const double a2 = (double)v[k].x * v[k].x + (double)v[k].y * v[k].y;
+        s12 += cf.ld<double>(CD::voff(), CD::soff(k)) * a2;
+        s21 += cf.ld<double>(CD::voff(), CD::soff(k) + ROW) * a2;
+        gs.st(v[k], CV::voff(), CV::soff(k));
+    }
+    const double w12 = block_sum<T>(s12, lds), w21 = block_sum<T>(s21, lds);
+    const double r12 = g.r12 * pin, r21 = g.r21 * pin;
+    const double n2 = (r12 + w12) / (r12 + r21 + w12 + w21 + g.inv_tau);
+    *kp = (g.gpe * n2 - g.gpa * (1.0 - n2)) * pin;
+    return n2;
+}
+
+// Adds g/2 .* x (simulate_disp_gain_mmf.jl:62) to the right-hand side v of
+// the stage input x kept by gain_pre.
+template <typename R, int N, int T>
+__device__ __forceinline__ void gain_post(const GainDev& g, C2<R>* v, const Fld& gs, double n2)
+{
+    using V = C2<R>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = N / T;
+    const Buf cf(g.coef, (uint32_t)(4 * N * sizeof(double)));
+    constexpr uint32_t ROW = N * sizeof(double);
+    const double n1 = 1.0 - n2;
+    drain_memory();
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const double ga = cf.ld<double>(CD::voff(), CD::soff(k) + 2 * ROW);
+        const double ge = cf.ld<double>(CD::voff(), CD::soff(k) + 3 * ROW);
+        const R hg = (R)(0.5 * (ge * n2 - ga * n1));
+        const V x = gs.template ld<V>(CV::voff(), CV::soff(k));
+        v[k] = V{v[k].x + hg * x.x, v[k].y + hg * x.y};
+    }
+}
+
+// |e| s / (atol + max(|u|, |w|) rtol), squared, for a real scalar entry.
+__device__ __forceinline__ double scaled_sq_scalar(double e, double u, double w, double s, double rtol, double atol)
+{
+    const double r = fabs(e) * s / (atol + fmax(fabs(u), fabs(w)) * rtol);
+    return r * r;
+}
+
 // Per-thread partial of sum_i |e_i s / (atol + max(|u_i|, |w_i|) rtol)|^2 in FP64.
 template <typename V>
 __device__ __forceinline__ double scaled_sq(V e, V u, V w, double s, double rtol, double atol)
@@ -162,10 +236,15 @@ __device__ __forceinline__ double scaled_sq(V e, V u, V w, double s, double rtol
 // (kept), else in the shared region the backward sweep recomputes into.  An
 // attempt that is rejected is overwritten by the next one; the seventh stage
 // (first same as last) is stage 0 of step n + 1.
-template <typename R, int N, int T, bool CKPT, bool SAVE = false>
+//
+// With GAIN the state is (Pp, x) (the pump + signal solve): *pump holds Pp(0)
+// on entry and Pp(L) on return, every stage adds the gain terms of
+// gain_pre/gain_post, and the error norms run over the N + 1 entries.
+template <typename R, int N, int T, bool CKPT, bool SAVE = false, bool GAIN = false>
 __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fld& slot0,
                           typename SmCtx<R, N, T>::FU& ut, const Buf& ckpt, const Buf& tape_u, const Buf& tape_h,
-                          int* ys_out, long long* nacc_out, long long* nrej_out, long long* nfe_out, void* lds)
+                          int* ys_out, long long* nacc_out, long long* nrej_out, long long* nfe_out, void* lds,
+                          double* pump = nullptr)
 {
     using V = C2<R>;
     using CV = Cyc<V, T>;
@@ -192,6 +271,12 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     auto th_at = [&](uint32_t slot) { return CKPT ? Fld{tape_h, slot * (uint32_t)(N * sizeof(R))} : nf; };
     V v[P];
     long long nfe = 0, nacc = 0, nrej = 0;
+    // pump + signal solves: pump state, its stage derivatives, the gain slot
+    // and the entry count of the error norms
+    [[maybe_unused]] double pp = GAIN ? *pump : 0.0, ppn = 0.0, kp[7] = {0, 0, 0, 0, 0, 0, 0};
+    [[maybe_unused]] const Fld gs = fld(LY::NW_FWD);
+    const bool lead = threadIdx.x == 0;
+    constexpr double NN = GAIN ? (double)(N + 1) : (double)N;
 
     // k0 = f(0, y0) and the starting-step norms
     {
@@ -199,7 +284,12 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
 #pragma unroll
         for (int k = 0; k < P; ++k) v[k] = y.template ld<V>(vo, CV::soff(k));
     }
-    sm_rhs<R, N, T, CKPT>(v, 0.0, a.s, tb, ut, tu_at(tape_slot(0, 0)), th_at(tape_slot(0, 0)), lds);
+    {
+        [[maybe_unused]] double n2 = 0;
+        if constexpr (GAIN) n2 = gain_pre<R, N, T>(a.gain, v, gs, pp, &kp[0], lds);
+        sm_rhs<R, N, T, CKPT>(v, 0.0, a.s, tb, ut, tu_at(tape_slot(0, 0)), th_at(tape_slot(0, 0)), lds);
+        if constexpr (GAIN) gain_post<R, N, T>(a.gain, v, gs, n2);
+    }
     nfe++;
     double s0 = 0, s1 = 0;
     {
@@ -212,8 +302,12 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
             s0 += scaled_sq(u, u, z0, 1.0, rtol, atol);
             s1 += scaled_sq(v[k], u, z0, 1.0, rtol, atol);
         }
+        if (GAIN && lead) {
+            s0 += scaled_sq_scalar(pp, pp, 0.0, 1.0, rtol, atol);
+            s1 += scaled_sq_scalar(kp[0], pp, 0.0, 1.0, rtol, atol);
+        }
     }
-    const double d0 = sqrt(block_sum<T>(s0, lds) / N), d1 = sqrt(block_sum<T>(s1, lds) / N);
+    const double d0 = sqrt(block_sum<T>(s0, lds) / NN), d1 = sqrt(block_sum<T>(s1, lds) / NN);
     double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
     dt0 = fmin(dt0, L);
     {
@@ -226,7 +320,13 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
             v[k] = V{u.x + (R)dt0 * f0.x, u.y + (R)dt0 * f0.y};
         }
     }
-    sm_rhs<R, N, T, false>(v, dt0, a.s, tb, ut, nf, nf, lds);
+    [[maybe_unused]] double kp1 = 0;
+    {
+        [[maybe_unused]] double n2 = 0;
+        if constexpr (GAIN) n2 = gain_pre<R, N, T>(a.gain, v, gs, pp + dt0 * kp[0], &kp1, lds);
+        sm_rhs<R, N, T, false>(v, dt0, a.s, tb, ut, nf, nf, lds);
+        if constexpr (GAIN) gain_post<R, N, T>(a.gain, v, gs, n2);
+    }
     nfe++;
     double s2 = 0;
     {
@@ -237,8 +337,9 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
             const V f0 = k0.template ld<V>(vo, CV::soff(k));
             s2 += scaled_sq(V{v[k].x - f0.x, v[k].y - f0.y}, u, V{(R)0, (R)0}, 1.0, rtol, atol);
         }
+        if (GAIN && lead) s2 += scaled_sq_scalar(kp1 - kp[0], pp, 0.0, 1.0, rtol, atol);
     }
-    const double d2 = sqrt(block_sum<T>(s2, lds) / N) / dt0;
+    const double d2 = sqrt(block_sum<T>(s2, lds) / NN) / dt0;
     const double mx = fmax(d1, d2);
     const double dt1 = mx <= 1e-15 ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / mx, 1.0 / 6.0);
     double h = fmin(fmin(100 * dt0, dt1), L);
@@ -302,7 +403,22 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
                 for (int k = 0; k < P; ++k) yn.st(v[k], vo, CV::soff(k));
             }
             const uint32_t ts = i < 6 ? tape_slot(nacc, i) : tape_slot(nacc + 1, 0);
+            [[maybe_unused]] double n2 = 0;
+            if constexpr (GAIN) {
+                double sp = 0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    if (j < i) sp += TB::a_at(i, j) * kp[j];
+                const double pin = pp + h * sp;
+                if (i == 6) ppn = pin;
+                double kpi = 0;
+                n2 = gain_pre<R, N, T>(a.gain, v, gs, pin, &kpi, lds);
+#pragma unroll
+                for (int j = 1; j < 7; ++j)
+                    if (j == i) kp[j] = kpi;
+            }
             sm_rhs<R, N, T, CKPT>(v, z + TB::c[i] * h, a.s, tb, ut, tu_at(ts), th_at(ts), lds);
+            if constexpr (GAIN) gain_post<R, N, T>(a.gain, v, gs, n2);
             const Fld ki = fld(kslot(i));
 #pragma unroll
             for (int k = 0; k < P; ++k) ki.st(v[k], vo, CV::soff(k));
@@ -323,8 +439,14 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
                 se += scaled_sq(e, y.template ld<V>(vo, CV::soff(k)), yn.template ld<V>(vo, CV::soff(k)), fabs(h),
                                 rtol, atol);
             }
+            if (GAIN && lead) {
+                double ep = 0;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) ep += TB::bt[j] * kp[j];
+                se += scaled_sq_scalar(ep, pp, ppn, fabs(h), rtol, atol);
+            }
         }
-        const double eest = sqrt(block_sum<T>(se, lds) / N);
+        const double eest = sqrt(block_sum<T>(se, lds) / NN);
         if (!isfinite(eest)) {
             status = 1;
             break;
@@ -355,6 +477,10 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
             const int t1 = sk0;
             sk0 = sk6;
             sk6 = t1;
+            if constexpr (GAIN) {
+                pp = ppn;
+                kp[0] = kp[6];
+            }
             h = h / q;
             done = last;
         } else {
@@ -366,6 +492,10 @@ __device__ int ts_forward(const TsArgs<R>& a, const SmTables<R, N>& tb, const Fl
     *nacc_out = nacc;
     *nrej_out = nrej;
     *nfe_out = nfe;
+    if constexpr (GAIN) {
+        *pump = pp;
+        if (!isfinite(pp) && status == 0) status = 1;
+    }
     return status;
 }
 
@@ -402,6 +532,62 @@ __global__ __launch_bounds__(T) void k_tsit5_single(TsArgs<R> a)
         a.stats[sol * 3 + 0] = nacc;
         a.stats[sol * 3 + 1] = nrej;
         a.stats[sol * 3 + 2] = nfe;
+    }
+    drain_memory();
+    int bad = status;
+    const Fld y{sl, (uint32_t)ys * (uint32_t)(N * sizeof(V))};
+    const Buf out(a.s.out + sol * N, N * sizeof(V));
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        R c, s;
+        cis_arg(tb.disp.template ld<double>(CD::voff(), CD::soff(k)), a.length(sol), c, s);
+        const V x = y.template ld<V>(vo, CV::soff(k));
+        const V o = V{x.x * c - x.y * s, x.x * s + x.y * c};
+        if (!(isfinite(o.x) && isfinite(o.y))) bad |= 1;
+        out.st(o, vo, CV::soff(k));
+    }
+    if (bad) atomicOr(a.s.status + sol, bad);
+}
+
+// Pump + signal solve (solve_disp_gain_smf, simulate_disp_gain_mmf.jl:200-239):
+// one workgroup per solve, state (Pp, x), output cis(D L) x(L) and Pp(L).
+// The slice holds the forward fields plus the gain slot.
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_tsit5_gain(TsArgs<R> a)
+{
+    using CX = SmCtx<R, N, T>;
+    using V = C2<R>;
+    using LY = TsLayout<R, N, T>;
+    using CV = Cyc<V, T>;
+    using CD = Cyc<double, T>;
+    constexpr int P = N / T;
+    constexpr int NW = LY::NW_FWD + 1;
+    __shared__ __align__(16) unsigned char lds[SmLayout<R, N, T>::LDS_BYTES];
+    const size_t sol = blockIdx.x;
+    const SmTables<R, N> tb(a.s, FftPlan<N, T>::TW_SIZE);
+    V* slice = a.s.work + sol * (size_t)NW * N;
+    const Buf sl(slice, (uint32_t)((size_t)NW * N * sizeof(V)));
+    const Fld base{sl, 0};
+    typename CX::FU ut(Buf(slice + (size_t)LY::CORE_FWD * N, N * sizeof(V)));
+    const uint32_t vo = CV::voff();
+    {
+        const Buf u0(a.s.u0 + sol * N, N * sizeof(V));
+        const Fld y{sl, (uint32_t)LY::Y * (uint32_t)(N * sizeof(V))};
+#pragma unroll
+        for (int k = 0; k < P; ++k) y.st(u0.ld<V>(vo, CV::soff(k)), vo, CV::soff(k));
+    }
+    drain_memory();
+    int ys;
+    long long nacc, nrej, nfe;
+    double pump = a.gain.pump0[sol];
+    const Buf none(a.s.u0, 0);
+    const int status = ts_forward<R, N, T, false, false, true>(a, tb, base, ut, none, none, none, &ys, &nacc, &nrej,
+                                                               &nfe, lds, &pump);
+    if (threadIdx.x == 0) {
+        a.stats[sol * 3 + 0] = nacc;
+        a.stats[sol * 3 + 1] = nrej;
+        a.stats[sol * 3 + 2] = nfe;
+        a.gain.pumpL[sol] = pump;
     }
     drain_memory();
     int bad = status;

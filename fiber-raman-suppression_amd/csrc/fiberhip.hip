@@ -1383,6 +1383,83 @@ int fh_forward_saveat(fh_plan p, int32_t batch, const double* u0, double* uL, in
     return finish_call(p, batch, status, stats, false);
 }
 
+int fh_forward_gain(fh_plan p, int32_t batch, const fh_gain_desc* g, const double* pump0, const double* u0,
+                    double* uL, double* pumpL, fh_stats* stats)
+{
+    if (!p || !g || !pump0 || !u0 || !uL || !pumpL) return fail(FH_EINVAL, "plan, gain, pump or field pointer is NULL");
+    if (!g->w12 || !g->w21 || !g->ga || !g->ge) return fail(FH_EINVAL, "gain coefficient array is NULL");
+    if (!p->adaptive || p->d.modes != 1 || p->mm || !fh::gain_supported(p->d.nt))
+        return fail(FH_EUNSUPPORTED, "pump + signal solves need a single-mode adaptive (FH_SCHEME_TSIT5) plan with "
+                                     "256 <= nt <= 16384");
+    if (batch < 1 || batch > p->d.max_batch)
+        return fail(FH_EINVAL, "batch must lie in [1, max_batch=" + std::to_string(p->d.max_batch) + "]");
+    const size_t nt = (size_t)p->d.nt, n = (size_t)batch * nt;
+    const bool fp64 = p->d.precision == FH_PREC_FP64;
+    if ((size_t)fh::gain_fields((int)nt, fp64) * nt > p->work_per_solve)
+        return fail(FH_EINVAL, "internal: plan slice too small for the gain slot");
+    std::vector<double> coef(4 * nt);
+    for (size_t k = 0; k < nt; ++k) {
+        coef[k] = g->w12[k];
+        coef[nt + k] = g->w21[k];
+        coef[2 * nt + k] = g->ga[k];
+        coef[3 * nt + k] = g->ge[k];
+    }
+    for (double c : coef)
+        if (!std::isfinite(c)) return fail(FH_EINVAL, "gain coefficients must be finite");
+    for (double c : {g->r12, g->r21, g->inv_tau, g->gpa, g->gpe})
+        if (!std::isfinite(c)) return fail(FH_EINVAL, "gain coefficients must be finite");
+    for (int b = 0; b < batch; ++b)
+        if (!std::isfinite(pump0[b])) return fail(FH_EINVAL, "pump power must be finite");
+    double norm2 = 0.0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        if (!std::isfinite(u0[i])) return fail(FH_EINVAL, "input spectrum must be finite");
+        norm2 += u0[i] * u0[i];
+    }
+    if (!(norm2 > 0)) return fail(FH_EINVAL, "input spectrum must be nonzero");
+    FH_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    TmpBuf dcoef, dp0, dpL;
+    FH_HIP(hipMalloc(&dcoef.p, sizeof(double) * 4 * nt));
+    FH_HIP(hipMalloc(&dp0.p, sizeof(double) * (size_t)batch));
+    FH_HIP(hipMalloc(&dpL.p, sizeof(double) * (size_t)batch));
+    FH_HIP(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * 4 * nt, hipMemcpyHostToDevice, s));
+    FH_HIP(hipMemcpyAsync(dp0.p, pump0, sizeof(double) * (size_t)batch, hipMemcpyHostToDevice, s));
+    int rc = upload_field(fp64, p->u_in.p, u0, n, s);
+    if (rc) return rc;
+    FH_HIP(hipMemsetAsync(p->status.p, 0, sizeof(int) * (size_t)batch, s));
+    fh::GainDev gd;
+    gd.coef = (const double*)dcoef.p;
+    gd.r12 = g->r12;
+    gd.r21 = g->r21;
+    gd.inv_tau = g->inv_tau;
+    gd.gpa = g->gpa;
+    gd.gpe = g->gpe;
+    gd.pump0 = (const double*)dp0.p;
+    gd.pumpL = (double*)dpL.p;
+    hipError_t e;
+    if (fp64) {
+        fh::SmArgs<double> a = base_args<double>(p);
+        a.u0 = (const fh::C2<double>*)p->u_in.p;
+        a.out = (fh::C2<double>*)p->u_out.p;
+        fh::TsArgs<double> ta = ts_args<double>(p, a);
+        ta.gain = gd;
+        e = fh::launch_tsit5_gain((int)nt, ta, batch, s);
+    } else {
+        fh::SmArgs<float> a = base_args<float>(p);
+        a.u0 = (const fh::C2<float>*)p->u_in.p;
+        a.out = (fh::C2<float>*)p->u_out.p;
+        fh::TsArgs<float> ta = ts_args<float>(p, a);
+        ta.gain = gd;
+        e = fh::launch_tsit5_gain((int)nt, ta, batch, s);
+    }
+    if (e != hipSuccess) return fail(FH_EDEVICE, std::string("gain launch: ") + hipGetErrorString(e));
+    std::vector<int> status(batch);
+    FH_HIP(hipMemcpyAsync(status.data(), p->status.p, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(pumpL, dpL.p, sizeof(double) * (size_t)batch, hipMemcpyDeviceToHost, s));
+    if ((rc = download_field(fp64, uL, p->u_out.p, n, s))) return rc;
+    return finish_call(p, batch, status, stats, false);
+}
+
 int fh_field_metrics(fh_plan p, int32_t count, const double* spectra, const double* omega, double dt_ps, double* out)
 {
     if (!p || !spectra || !omega || !out) return fail(FH_EINVAL, "plan, spectra, omega or out pointer is NULL");

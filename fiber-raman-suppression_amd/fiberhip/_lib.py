@@ -135,6 +135,20 @@ class RequestResult(ctypes.Structure):
     ]
 
 
+class GainDesc(ctypes.Structure):
+    _fields_ = [
+        ("w12", ctypes.POINTER(ctypes.c_double)),
+        ("w21", ctypes.POINTER(ctypes.c_double)),
+        ("ga", ctypes.POINTER(ctypes.c_double)),
+        ("ge", ctypes.POINTER(ctypes.c_double)),
+        ("r12", ctypes.c_double),
+        ("r21", ctypes.c_double),
+        ("inv_tau", ctypes.c_double),
+        ("gpa", ctypes.c_double),
+        ("gpe", ctypes.c_double),
+    ]
+
+
 FH_REQ_FORWARD = 0
 FH_REQ_GRADIENT = 1
 
@@ -156,6 +170,8 @@ _SIGNATURES = {
     "fh_forward": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),
     "fh_forward_saveat": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_int32, _DP, _DP, ctypes.POINTER(Stats)],
                           ctypes.c_int),
+    "fh_forward_gain": ([_VP, ctypes.c_int32, ctypes.POINTER(GainDesc), _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)],
+                        ctypes.c_int),
     "fh_field_metrics": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_double, _DP], ctypes.c_int),
     "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),

@@ -171,6 +171,32 @@ int fh_forward(fh_plan plan, int32_t batch, const double *u0, double *uL, fh_sta
 int fh_forward_saveat(fh_plan plan, int32_t batch, const double *u0, double *uL, int32_t nz, const double *zsave,
                       double *saved, fh_stats *stats);
 
+/* Gain model of the pump + signal propagation (disp_gain_smf!,
+ * simulate_disp_gain_mmf.jl:21-63, with calculate_gain_YDFA,
+ * gain.jl:148-167), folded into coefficients on |u_k|^2 of the field in FFT
+ * order (nt entries each):
+ *   W12 = sum_k w12[k] |u_k|^2,  W21 = sum_k w21[k] |u_k|^2
+ *   R12 = r12 Pp,  R21 = r21 Pp
+ *   n2 = (R12 + W12) / (R12 + R21 + W12 + W21 + inv_tau),  n1 = 1 - n2
+ *   gain g_k = ge[k] n2 - ga[k] n1 [1/m],  pump gain gP = gpe n2 - gpa n1.
+ * The constant model (compute_gain(uω, pGain::Number, Pp), :76-80) is
+ * ga = -pGain, ge = w12 = w21 = 0, r12 = r21 = 0, inv_tau = 1, gpa = 1,
+ * gpe = 0.  Host arrays, read during the call. */
+typedef struct {
+    const double *w12, *w21, *ga, *ge;
+    double r12, r21, inv_tau, gpa, gpe;
+} fh_gain_desc;
+
+/* solve_disp_gain_smf (simulate_disp_gain_mmf.jl:200-239): adaptive Tsit5
+ * on the state [Pp; x] (nt + 1 entries, the error norm over all of them)
+ * with dx = disp_mmf!(x) + g/2 .* x and dPp = gP Pp, from 0 to the plan's
+ * length, for `batch` launches.  Single-mode FH_SCHEME_TSIT5 plans with
+ * 256 <= nt <= 16384 (FH_EUNSUPPORTED otherwise); the plan's rtol/atol
+ * (the reference uses reltol 1e-5 and the default abstol 1e-6).  pump0,
+ * pumpL: batch pump powers [W] at 0 and L; u0, uL as in fh_forward. */
+int fh_forward_gain(fh_plan plan, int32_t batch, const fh_gain_desc *gain, const double *pump0, const double *u0,
+                    double *uL, double *pumpL, fh_stats *stats);
+
 /* Same solve on device-resident arrays in the plan's precision (complex128
  * for FH_PREC_FP64, complex64 for FH_PREC_FP32_BULK), enqueued on `stream`
  * (a hipStream_t; NULL = the context stream).  Does not synchronise. */

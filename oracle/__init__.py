@@ -272,7 +272,58 @@ def tsit5_adjoint(fib: Fiber, length: float, hs, u0, lam_out, every: int = 50) -
     return out.T
 
 
-__all__ = ["Fiber", "build", "lib", "fft", "rhs", "rhs_vjp", "rk4_forward", "rk4_adjoint",
+class _Gain(ctypes.Structure):
+    _fields_ = [("w12", ctypes.POINTER(ctypes.c_double)), ("w21", ctypes.POINTER(ctypes.c_double)),
+                ("ga", ctypes.POINTER(ctypes.c_double)), ("ge", ctypes.POINTER(ctypes.c_double)),
+                ("r12", ctypes.c_double), ("r21", ctypes.c_double), ("inv_tau", ctypes.c_double),
+                ("gpa", ctypes.c_double), ("gpe", ctypes.c_double)]
+
+
+class Gain:
+    """Gain coefficients (per-bin arrays in FFT order, scalars) of the pump +
+    signal model in the C layout; `c` is any object with the fields w12, w21,
+    ga, ge, r12, r21, inv_tau, gpa, gpe."""
+
+    def __init__(self, c):
+        self._a = [np.ascontiguousarray(getattr(c, k), dtype=np.float64) for k in ("w12", "w21", "ga", "ge")]
+        self._s = _Gain(*[_dp(a) for a in self._a], float(c.r12), float(c.r21), float(c.inv_tau), float(c.gpa),
+                        float(c.gpe))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self._s)
+
+
+def gain_rhs(fib: Fiber, gain: Gain, z: float, pump: float, x):
+    """One evaluation of disp_gain_smf! (simulate_disp_gain_mmf.jl:21-63) -> (dPp, dx)."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).reshape(-1))
+    dx = np.zeros_like(x)
+    dp = np.zeros(2)
+    _check(lib().orc_gain_rhs(fib.ref, gain.ref, ctypes.c_double(z), ctypes.c_double(pump),
+                              _dp(x.view(np.float64)), _dp(dp), _dp(dx.view(np.float64))), "orc_gain_rhs")
+    return complex(dp[0], dp[1]), dx[:, None]
+
+
+def gain_forward(fib: Fiber, gain: Gain, length: float, u0, pump0: float, rtol: float = 1e-5, atol: float = 1e-6,
+                 max_steps: int = 100000):
+    """solve_disp_gain_smf (simulate_disp_gain_mmf.jl:200-239): Tsit5 on [Pp; x]
+    -> (lab output spectrum (nt, 1), Pp(L), accepted step sizes, stats)."""
+    u0 = _field(u0)
+    out = np.zeros_like(u0)
+    pl = np.zeros(1)
+    hs = np.zeros(max_steps)
+    st = np.zeros(3, dtype=np.int64)
+    rc = lib().orc_gain_forward(fib.ref, gain.ref, ctypes.c_double(length), ctypes.c_double(rtol),
+                                ctypes.c_double(atol), int(max_steps), ctypes.c_double(pump0),
+                                _dp(u0.view(np.float64)), _dp(out.view(np.float64)), _dp(pl), _dp(hs),
+                                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    if rc == -3:
+        raise RuntimeError("orc_gain_forward: max_steps reached")
+    _check(rc, "orc_gain_forward")
+    return out.T, float(pl[0]), hs[: st[0]].copy(), st
+
+
+__all__ = ["Fiber", "build", "lib", "Gain", "gain_rhs", "gain_forward", "fft", "rhs", "rhs_vjp", "rk4_forward", "rk4_adjoint",
            "adjoint_rhs", "continuous_adjoint", "tsit5_tableau", "tsit5_dense_weights", "tsit5_forward", "tsit5_replay",
            "tsit5_adjoint", "np_ref"]
 _ = os  # keep import for callers that patch paths

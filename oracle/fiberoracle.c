@@ -900,6 +900,181 @@ int orc_tsit5_forward(const orc_fiber *f, double length_m, double rtol, double a
     return orc_tsit5_forward_save(f, length_m, rtol, atol, max_steps, u0, out, hs, stats, 0, NULL, NULL);
 }
 
+/* ------------------------------------------------ pump + signal with gain */
+
+/* Gain model of disp_gain_smf! (simulate_disp_gain_mmf.jl:21-63) with the
+ * rate-equation gain of calculate_gain_YDFA (gain.jl:150-166), its inputs
+ * folded into per-bin and scalar coefficients by the caller:
+ *   W12 = sum_k w12[k] |u_k|^2,  W21 = sum_k w21[k] |u_k|^2   (gain.jl:154-155
+ *         with Ps = fftshift(|u Nt dt|^2) rep, psd_from_uω :115-120)
+ *   R12 = r12 Pp, R21 = r21 Pp                                  (:151-152)
+ *   n2 = (R12 + W12) / (R12 + R21 + W12 + W21 + inv_tau), n1 = 1 - n2  (:157-158)
+ *   g_k = ge[k] n2 - ga[k] n1   (Gamma_s (sigma_es n2 - sigma_as n1) rho, :160)
+ *   gP  = gpe n2 - gpa n1       (Gamma_p (sigma_ep n2 - sigma_ap n1) rho, :162)
+ * Arrays in FFT order (the reference computes on the shifted grid and
+ * ifftshifts the gain back, simulate_disp_gain_mmf.jl:98-100).  The
+ * constant model (compute_gain(uω, pGain::Number, Pp), :76-80: g = pGain,
+ * gP = -1) is ga = -pGain, ge = 0, gpa = 1, gpe = 0, every rate 0 and
+ * inv_tau = 1 (n2 = 0). */
+typedef struct {
+    const double *w12, *w21, *ga, *ge;
+    double r12, r21, inv_tau, gpa, gpe;
+} orc_gain;
+
+/* disp_gain_smf! (simulate_disp_gain_mmf.jl:21-63) for M = 1: state
+ * (Pp, x).  dx = disp_mmf!(x) + g/2 .* x (:62), dPp = gP Pp (:61).  |u|^2 of
+ * the lab spectrum cis(Dz) x equals |x|^2. */
+static void gain_rhs_eval(const orc_fiber *f, const orc_gain *g, orc_ws *ws, double z, cplx pp, const cplx *x,
+                          cplx *dpp, cplx *dx)
+{
+    const int nt = f->nt;
+    rhs_eval(f, ws, z, x, dx);
+    double w12 = 0, w21 = 0;
+    for (int k = 0; k < nt; ++k) {
+        const double a2 = creal(x[k]) * creal(x[k]) + cimag(x[k]) * cimag(x[k]);
+        w12 += g->w12[k] * a2;
+        w21 += g->w21[k] * a2;
+    }
+    const cplx r12 = g->r12 * pp, r21 = g->r21 * pp;
+    const cplx n2 = (r12 + w12) / (r12 + r21 + w12 + w21 + g->inv_tau), n1 = 1.0 - n2;
+    for (int k = 0; k < nt; ++k) dx[k] += 0.5 * (g->ge[k] * n2 - g->ga[k] * n1) * x[k];
+    *dpp = (g->gpe * n2 - g->gpa * n1) * pp;
+}
+
+/* sum over the state (Pp, x) of |e s / (atol + max(|u0|, |u1|) rtol)|^2 */
+static double gain_scaled_sq(cplx e, cplx u0, cplx u1, double rtol, double atol, double scale)
+{
+    const double au = cabs(u0), bu = cabs(u1);
+    const double r = cabs(e) * scale / (atol + (au > bu ? au : bu) * rtol);
+    return r * r;
+}
+
+/* solve_disp_gain_smf (simulate_disp_gain_mmf.jl:200-239): Tsit5 on the
+ * state [Pp; x] of nt + 1 entries from 0 to L, with the controller and
+ * starting step of orc_tsit5_forward_save and the RMS norm over all nt + 1
+ * entries (OrdinaryDiffEq's norm of the concatenated state, :212-213).
+ * out = cis(D L) x(L), *pump_out = real(Pp(L)).  Returns as
+ * orc_tsit5_forward_save. */
+int orc_gain_forward(const orc_fiber *f, const orc_gain *g, double length_m, double rtol, double atol,
+                     int max_steps, double pump0, const double *u0, double *out, double *pump_out, double *hs,
+                     int64_t *stats)
+{
+    if (f->m != 1 || length_m <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0) return -1;
+    const size_t nf = (size_t)f->nt;
+    const double nn = (double)(nf + 1);
+    orc_ws ws;
+    if (ws_init(&ws, f->nt, 1)) return -1;
+    cplx *k = malloc(sizeof(cplx) * nf * 7), *y = malloc(sizeof(cplx) * nf);
+    cplx *ynew = malloc(sizeof(cplx) * nf), *tmp = malloc(sizeof(cplx) * nf);
+    if (!k || !y || !ynew || !tmp) { free(k); free(y); free(ynew); free(tmp); ws_free(&ws); return -1; }
+    memcpy(y, u0, sizeof(cplx) * nf);
+    cplx p = pump0, pnew = 0, kp[7];
+    int64_t nacc = 0, nrej = 0, nfe = 0;
+    int rc = 0;
+    gain_rhs_eval(f, g, &ws, 0.0, p, y, &kp[0], k);
+    nfe++;
+    double s0 = gain_scaled_sq(p, p, 0, rtol, atol, 1.0), s1 = gain_scaled_sq(kp[0], p, 0, rtol, atol, 1.0);
+    for (size_t o = 0; o < nf; ++o) {
+        s0 += gain_scaled_sq(y[o], y[o], 0, rtol, atol, 1.0);
+        s1 += gain_scaled_sq(k[o], y[o], 0, rtol, atol, 1.0);
+    }
+    const double d0 = sqrt(s0 / nn), d1 = sqrt(s1 / nn);
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    if (dt0 > length_m) dt0 = length_m;
+    for (size_t o = 0; o < nf; ++o) tmp[o] = y[o] + dt0 * k[o];
+    cplx p1 = p + dt0 * kp[0], kp1;
+    gain_rhs_eval(f, g, &ws, dt0, p1, tmp, &kp1, ynew);
+    nfe++;
+    double s2 = gain_scaled_sq(kp1 - kp[0], p, 0, rtol, atol, 1.0);
+    for (size_t o = 0; o < nf; ++o) s2 += gain_scaled_sq(ynew[o] - k[o], y[o], 0, rtol, atol, 1.0);
+    const double d2 = sqrt(s2 / nn) / dt0;
+    const double mx = d1 > d2 ? d1 : d2;
+    const double dt1 = mx <= 1e-15 ? (dt0 * 1e-3 > 1e-6 ? dt0 * 1e-3 : 1e-6) : pow(0.01 / mx, 1.0 / 6.0);
+    double h = 100 * dt0;
+    if (dt1 < h) h = dt1;
+    if (length_m < h) h = length_m;
+    double z = 0.0, qold = TS_QOLDINIT;
+    int done = 0;
+    while (!done) {
+        if (nacc >= max_steps) { rc = -3; break; }
+        int last = 0;
+        if (z + h >= length_m) { h = length_m - z; last = 1; }
+        for (int i = 1; i < 7; ++i) {
+            cplx *dst = i == 6 ? ynew : tmp;
+            cplx sp = 0;
+            for (int j = 0; j < i; ++j) sp += TS_A[i][j] * kp[j];
+            const cplx pin = p + h * sp;
+            if (i == 6) pnew = pin;
+            for (size_t o = 0; o < nf; ++o) {
+                cplx s = 0;
+                for (int j = 0; j < i; ++j) s += TS_A[i][j] * k[(size_t)j * nf + o];
+                dst[o] = y[o] + h * s;
+            }
+            gain_rhs_eval(f, g, &ws, z + TS_C[i] * h, pin, dst, &kp[i], k + (size_t)i * nf);
+        }
+        nfe += 6;
+        cplx ep = 0;
+        for (int j = 0; j < 7; ++j) ep += TS_BT[j] * kp[j];
+        double se = gain_scaled_sq(ep, p, pnew, rtol, atol, fabs(h));
+        for (size_t o = 0; o < nf; ++o) {
+            cplx s = 0;
+            for (int j = 0; j < 7; ++j) s += TS_BT[j] * k[(size_t)j * nf + o];
+            se += gain_scaled_sq(s, y[o], ynew[o], rtol, atol, fabs(h));
+        }
+        const double e = sqrt(se / nn);
+        if (!isfinite(e)) { rc = -2; break; }
+        const double q11 = pow(e, TS_BETA1);
+        if (e <= 1.0) {
+            double q = q11 / pow(qold, TS_BETA2);
+            q = q / TS_SAFETY;
+            if (q < 1.0 / TS_QMAX) q = 1.0 / TS_QMAX;
+            if (q > 1.0 / TS_QMIN) q = 1.0 / TS_QMIN;
+            qold = e > TS_QOLDINIT ? e : TS_QOLDINIT;
+            if (hs) hs[nacc] = h;
+            nacc++;
+            z += h;
+            memcpy(y, ynew, sizeof(cplx) * nf);
+            memcpy(k, k + 6 * nf, sizeof(cplx) * nf);
+            p = pnew;
+            kp[0] = kp[6];
+            h = h / q;
+            done = last;
+        } else {
+            nrej++;
+            double sh = q11 / TS_SAFETY;
+            if (sh > 1.0 / TS_QMIN) sh = 1.0 / TS_QMIN;
+            h = h / sh;
+        }
+    }
+    if (stats) { stats[0] = nacc; stats[1] = nrej; stats[2] = nfe; }
+    if (rc == 0) {
+        cplx *o = (cplx *)out;
+        for (size_t i = 0; i < nf; ++i) {
+            o[i] = cis(f->disp[i] * length_m) * y[i];
+            if (!isfinite(creal(o[i])) || !isfinite(cimag(o[i]))) rc = -2;
+        }
+        *pump_out = creal(p);
+        if (!isfinite(*pump_out)) rc = -2;
+    }
+    free(k); free(y); free(ynew); free(tmp);
+    ws_free(&ws);
+    return rc;
+}
+
+/* One evaluation of the pump + signal right-hand side, for tests. */
+int orc_gain_rhs(const orc_fiber *f, const orc_gain *g, double z, double pump, const double *x, double *dpump,
+                 double *dx)
+{
+    orc_ws ws;
+    if (f->m != 1 || ws_init(&ws, f->nt, 1)) return -1;
+    cplx dp;
+    gain_rhs_eval(f, g, &ws, z, pump, (const cplx *)x, &dp, (cplx *)dx);
+    dpump[0] = creal(dp);
+    dpump[1] = cimag(dp);
+    ws_free(&ws);
+    return 0;
+}
+
 /* One Tsit5 step with a given h (no error control), 6 stages; b = row 7 of A.
  * If ys != NULL the stage inputs Y_1..Y_6 are stored there. */
 static void tsit5_step_fixed(const orc_fiber *f, orc_ws *ws, double z, double h, cplx *y, cplx *k, cplx *tmp,
