@@ -22,6 +22,17 @@ import time
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 
+# The serving leg runs one launch per lane concurrently, each on its own
+# stream; HIP maps streams onto hardware queues per process (4 by default,
+# and environments may pin that value), and work of streams sharing a queue
+# runs one after another: a short request on a free lane waited 4.5 s behind
+# a long launch with 4 queues and 4 lanes, and 1 ms with 20 queues and 16
+# lanes.  Four queues beyond the lanes keep the null and context streams
+# apart too.  HIP reads the value once, when it initialises, so it is set
+# here, before any HIP call, overriding an inherited value.
+SERVE_LANES = 16
+os.environ["GPU_MAX_HW_QUEUES"] = str(SERVE_LANES + 4)
+
 import numpy as np  # noqa: E402
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense vector/matrix peaks
@@ -201,6 +212,47 @@ def time_sweep(problem, precision, candidates, per_launch, k, warmup, mask, rtol
     return dict(ms=float(np.mean(sweeps)), stats=np.concatenate(stats), cost=np.concatenate(costs))
 
 
+def gain_workload(nt: int = 4096):
+    """YDFA amplifier (solve_disp_gain_smf): 1030 nm, Nt samples over 20 ps,
+    1 m of Yb fiber with gamma 4.7e-3 /W/m, beta2 20 ps^2/km, fR 0.18, a
+    300 fs Gaussian at 50 mW average (80 MHz) and a 0.5 W co-propagating pump
+    at 976 nm; the reference's YDFA parameters (gain.jl:30-75) with its Yb
+    cross-section tables (tests/golden/yb_cross_sections.json, data)."""
+    from fiberhip import gain as G
+    from fiberhip import problem as P
+
+    xs = json.loads((ROOT / "tests" / "golden" / "yb_cross_sections.json").read_text())
+    sim = P.get_disp_sim_params(1030e-9, 1, nt, 20.0 * nt / 4096, 3)
+    cs = G.get_ydfa_cross_sections(sim["fs"], xs["absorption"]["wavelength_nm"], xs["absorption"]["intensity"],
+                                   xs["emission"]["wavelength_nm"], xs["emission"]["intensity"], xs["scale_m2"])
+    p = G.get_YDFAParams(sim, cs)
+    _, u0 = P.get_initial_state([1.0], 0.05, 300e-15, 80e6, "gauss", sim)
+    fib = P.get_disp_fiber_params_user_defined(1.0, sim, fR=0.18, gamma_user=4.7e-3, betas_user=[20e-27, 0.0])
+    fib["gain_parameters"] = p
+    return sim, fib, u0, 0.5
+
+
+def time_gain(precision: str, batch: int, k: int, nt: int = 4096):
+    """Wall time of fh_forward_gain over `batch` launches (seeded GDD/TOD
+    phases on the signal), including host staging; returns per-call seconds
+    and the per-solve step counts of the last call."""
+    from fiberhip import gain as G
+
+    sim, fib, u0, pump = gain_workload(nt)
+    rng = np.random.default_rng(6)
+    w = 2 * np.pi * np.fft.fftfreq(nt, sim["Δt"]) * 1e-3  # rad/fs
+    launches_ = np.stack([u0 * np.exp(1j * (rng.uniform(-1e4, 1e4) * w ** 2 / 2
+                                            + rng.uniform(-5e5, 5e5) * w ** 3 / 6))[:, None]
+                          for _ in range(batch)])  # (batch, nt, 1)
+    G.solve_disp_gain_smf(launches_, fib, sim, pump_power=pump, precision=precision)  # warm-up
+    ts = []
+    for _ in range(k):
+        t = time.perf_counter()
+        res = G.solve_disp_gain_smf(launches_, fib, sim, pump_power=pump, precision=precision)
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), res
+
+
 SERVE_WINDOW_PS = {2048: 5.883, 4096: 11.766, 8192: 23.531, 16384: 47.061, 32768: 94.122}
 
 
@@ -239,7 +291,7 @@ def serving_problem(cache: dict, nt: int, modes: int):
     return cache[key]
 
 
-def run_serving(trace, precision, rate, cache, lanes=8, max_batch=32):
+def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, budget_gib=96):
     """Submit the trace with Poisson arrivals at `rate` requests/s from a feeder
     thread and wait for every request; returns per-request results."""
     import fiberhip
@@ -252,8 +304,12 @@ def run_serving(trace, precision, rate, cache, lanes=8, max_batch=32):
         u0 = p.uω0 * np.sqrt(r["peak"] / peak) * np.exp(1j * (basis @ r["coef"]))[:, None]
         inputs.append((p, u0, mask))
     ids, rejected = [None] * len(trace), {}
-    srv = fiberhip.Server(memory_budget_bytes=96 << 30, lanes=lanes, max_batch=max_batch, precision=precision,
+    srv = fiberhip.Server(memory_budget_bytes=int(budget_gib * (1 << 30)), lanes=lanes, max_batch=max_batch,
+                          precision=precision,
                           rtol=1e-6, atol=1e-6, multimode_step_m=5e-3, max_steps=20000)
+    # latency runs from each request's scheduled arrival (steady clock, the
+    # server's clock) to its completion, so time a submission waits counts
+    t0_ns = time.monotonic_ns()
 
     def feed():
         t0 = time.perf_counter()
@@ -273,13 +329,19 @@ def run_serving(trace, precision, rate, cache, lanes=8, max_batch=32):
     th.start()
     th.join()
     results = [None] * len(trace)
+    done = 0
     for i, rid in enumerate(ids):
         if rid is None:
             continue
         try:
             results[i] = srv.wait(rid)
+            results[i]["latency_ms"] = (results[i]["finish_ns"] - t0_ns) / 1e6 - arrivals[i] * 1e3
         except fiberhip.FiberHIPError as exc:
             results[i] = dict(error=str(exc))
+        done += 1
+        if done % max(1, len(trace) // 10) == 0:
+            print(f"bench: serving {precision}: {done} of {len(trace)} requests finished, "
+                  f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t_start
     srv.close()
     return results, rejected, wall
@@ -412,13 +474,17 @@ def main() -> None:
                          "and FP64; 0 recomputes every segment)")
     ap.add_argument("--fwd-batch", type=int, default=1024, help="solves per launch (SMF-28 0.5 m forward workload)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "serving"], default="all")
+    ap.add_argument("--only", choices=["all", "gradient", "forward", "multimode", "sweep", "gain", "serving"],
+                    default="all")
+    ap.add_argument("--gain-batch", type=int, default=256, help="amplifier solves per call (YDFA gain workload)")
     ap.add_argument("--sweep-candidates", type=int, default=256, help="adaptive candidate sweep size")
     ap.add_argument("--sweep-tape-segments", type=int, default=3,
                     help="accepted-step segments of 50 whose stage tape the sweep's forward pass keeps (0 recomputes "
                          "every segment)")
     ap.add_argument("--serve-requests", type=int, default=100, help="requests in the serving trace")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-budget-gib", type=float, default=96.0, help="device memory of the server's plans")
+    ap.add_argument("--serve-precisions", default="fp32,fp64", help="server precisions to run, comma-separated")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
@@ -611,12 +677,43 @@ def main() -> None:
                     "data": "synthetic", "config": {"workload": sweep["workload"]}, "sweep": sweep}
         else:
             line["sweep_smf28_8192"] = sweep
+    if args.only in ("all", "gain"):
+        nt = 4096
+        res = {p: time_gain(p, args.gain_batch, max(1, args.steps), nt) for p in ("fp32", "fp64")}
+        # algorithmic flops from the counted right-hand-side evaluations: the
+        # disp_mmf! structure plus 12 N for the gain (|x|^2, two rate sums,
+        # g_k and the g/2 x term)
+        nrhs = {p: res[p][1].stats["rhs_evals"] for p in res}
+        flops = {p: nrhs[p] * (rhs_flops(nt, 1) + 12 * nt) / args.gain_batch for p in res}
+        e32 = np.sum(np.abs(res["fp32"][1].output_spectrum) ** 2, axis=(1, 2))
+        e64 = np.sum(np.abs(res["fp64"][1].output_spectrum) ** 2, axis=(1, 2))
+        gain_line = {
+            "workload": f"YDFA amplifier (solve_disp_gain_smf): 1030 nm, Nt={nt}, 20 ps, 1 m Yb fiber, 300 fs "
+                        f"Gaussian 50 mW + 0.5 W pump, GDD/TOD launch phases, Tsit5 reltol 1e-5 on [Pp; x]",
+            "batch_solves_per_call": args.gain_batch,
+            "timing": "median wall time of the host call fh_forward_gain, host staging and copies included",
+            **{f"{p}_solves_per_s": args.gain_batch / res[p][0] for p in res},
+            **{f"{p}_ms_per_call": res[p][0] * 1e3 for p in res},
+            **{f"{p}_rhs_evals_per_solve": nrhs[p] / args.gain_batch for p in res},
+            **{f"{p}_roofline_frac": args.gain_batch * flops[p] / res[p][0] / 1e12 / PEAK_TFLOPS[p] for p in res},
+            "fp32_vs_fp64_max_rel_energy_err": float(np.max(np.abs(e32 / e64 - 1))),
+            "fp32_vs_fp64_max_rel_pump_err": float(np.max(np.abs(res["fp32"][1].pump_power /
+                                                                 res["fp64"][1].pump_power - 1))),
+        }
+        if args.only == "gain":
+            line = {"metric": "amplifier solves/sec (FP32-bulk, YDFA gain)", "value": gain_line["fp32_solves_per_s"],
+                    "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                    "ms_per_step": gain_line["fp32_ms_per_call"], "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                    "config": {"workload": gain_line["workload"]}, "gain": gain_line}
+        else:
+            line["gain_ydfa_4096"] = gain_line
     if args.only in ("all", "serving"):
         trace = serving_trace(args.serve_requests)
         cache = {}
         runs = {}
-        for precision in ("fp32", "fp64"):
-            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache)
+        for precision in args.serve_precisions.split(","):
+            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache, budget_gib=args.serve_budget_gib)
             ok = [r for r in res if r is not None and "error" not in r]
             lat = np.array([r["latency_ms"] for r in ok])
             runs[precision] = dict(results=res, rejected=rejected, wall_s=wall, served=len(ok),
@@ -628,7 +725,7 @@ def main() -> None:
             print(f"bench: serving {precision}: {len(ok)} served in {wall:.1f} s, p50 {runs[precision]['p50_ms']} ms, "
                   f"p99 {runs[precision]['p99_ms']} ms", file=sys.stderr, flush=True)
         errs = []
-        for a, b in zip(runs["fp32"]["results"], runs["fp64"]["results"]):
+        for a, b in zip(runs.get("fp32", {}).get("results", []), runs.get("fp64", {}).get("results", [])):
             if a and b and "error" not in a and "error" not in b:
                 ea, eb = np.sum(np.abs(a["uL"]) ** 2, axis=0), np.sum(np.abs(b["uL"]) ** 2, axis=0)
                 errs.append(float(np.max(np.abs(ea / eb - 1))))
@@ -637,8 +734,11 @@ def main() -> None:
                         f"Nt in 2^11..2^15, M in {{1,3,6,10}} (SMF-28 / 50 um GRIN HG), L log-uniform 0.1-5 m, "
                         f"peak 1-20 kW, GDD/TOD/FOD phases, 50% forward / 50% forward+adjoint; single mode adaptive "
                         f"Tsit5 rtol=atol=1e-6, multimode fixed 5 mm RK4IP steps",
-            **{f"{p}_{k}": runs[p][k] for p in ("fp32", "fp64")
+            **{f"{p}_{k}": runs[p][k] for p in runs
                for k in ("served", "failed", "p50_ms", "p99_ms", "solves_per_s", "wall_s", "mean_batch", "rejected")},
+            "lanes": SERVE_LANES, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+            "memory_budget_gib": args.serve_budget_gib,
+            "latency": "from each request's scheduled Poisson arrival to its completion",
             "fp32_vs_fp64_matched_requests": len(errs),
             "fp32_vs_fp64_max_rel_mode_energy_err": max(errs) if errs else None,
             "fp32_vs_fp64_p99_rel_mode_energy_err": float(np.percentile(errs, 99)) if errs else None,

@@ -2,7 +2,10 @@
 // independent forward / forward+adjoint solves (see fh_server_* in
 // include/fiberhip.h).
 //
-// One scheduler thread owns every HIP call of the server.  Requests that
+// One scheduler thread owns every HIP call of the server and releases the
+// request lock while it makes them (plan creation and release, staging,
+// launches, output scatter), so submissions never wait behind device work.
+// Requests that
 // share their operators form a batch (one fused launch, one workgroup per
 // solve, each solve with its own length); up to `lanes` batches run at once
 // on their own default-priority streams, each with an exclusive plan
@@ -17,6 +20,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -52,6 +57,7 @@ struct Fnv {
 struct OpSet {
     int nt = 0, modes = 0;
     bool grad = false;
+    int capacity = 0;  // solves per launch of this class (0 = not sized yet)
     std::vector<double> disp, gamma, raman, shock;
     std::vector<uint8_t> mask;
     double one_m_fr = 0;
@@ -62,6 +68,7 @@ struct Req {
     uint64_t key = 0;
     double length = 0;
     int nsteps = 0;
+    bool heavy = false;  // estimated work above kHeavyFlops
     std::vector<double> u0;
     double *uL = nullptr, *grad = nullptr, *cost = nullptr;
     int state = 0;  // 0 queued, 1 running, 2 finished
@@ -71,38 +78,63 @@ struct Req {
 struct PlanInst {
     fh_plan plan = nullptr;
     uint64_t key = 0;
+    int capacity = 1;
     bool busy = false;
     int64_t last_used = 0, bytes = 0;
 };
 
+// Pinned staging buffer of a lane.  Growing it retires the old buffer
+// instead of freeing it: hipHostFree waits for all work on the device, which
+// would stall the scheduler behind every running launch.  Retired buffers
+// (at most about the final size in total, growth is geometric) are freed
+// when the server is destroyed.
 template <typename T> struct Pinned {
     T* p = nullptr;
     size_t n = 0;
+    std::vector<T*> retired;
     int ensure(size_t want)
     {
         if (want <= n) return FH_OK;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipHostMalloc((void**)&p, sizeof(T) * want, hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
+        const size_t grow = std::max(want, 2 * n);
+        T* q = nullptr;
+        if (hipHostMalloc((void**)&q, sizeof(T) * grow, hipHostMallocDefault) != hipSuccess)
             return fhi::set_error(FH_ENOMEM, "hipHostMalloc of a staging buffer failed");
-        }
-        n = want;
+        if (p) retired.push_back(p);
+        p = q;
+        n = grow;
         return FH_OK;
     }
     void release()
     {
         if (p) (void)hipHostFree(p);
+        for (T* q : retired) (void)hipHostFree(q);
+        retired.clear();
         p = nullptr;
         n = 0;
     }
 };
 
+// Opt-in trace (environment FH_SERVER_TRACE set): scheduler steps that took
+// longer than 20 ms, to stderr.
+bool trace_on()
+{
+    static const bool on = std::getenv("FH_SERVER_TRACE") != nullptr;
+    return on;
+}
+
+void trace_slow(const char* what, int64_t t0, int detail)
+{
+    if (!trace_on()) return;
+    const int64_t dt = std::chrono::duration_cast<std::chrono::nanoseconds>(
+                           std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+    if (dt > 20000000) std::fprintf(stderr, "fh_server: %s took %.1f ms (%d)\n", what, dt / 1e6, detail);
+}
+
 struct Lane {
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr;
     bool busy = false;
+    bool heavy = false;  // running a batch of heavy requests
     bool grad = false;
     int modes = 1, nt = 0;
     PlanInst* pi = nullptr;
@@ -127,6 +159,7 @@ struct fh_server_s {
     std::unordered_map<uint64_t, OpSet> ops;
     std::list<PlanInst> plans;
     std::vector<Lane> lanes;
+    fhi::Arena arena;  // the memory budget, allocated once; plans sub-allocate from it
     int64_t next_id = 1, plan_bytes = 0;
     bool stop = false;
     std::thread worker;
@@ -136,6 +169,25 @@ namespace {
 
 // Largest fixed-step count of a multimode request.
 constexpr int kMaxMultimodeSteps = 1 << 22;
+
+// A launch holds its lane until its slowest solve ends.  Requests whose
+// estimated work exceeds kHeavyFlops (about ten seconds of one workgroup)
+// may occupy at most lanes - lanes/4 lanes, so light requests keep lanes to
+// run on while heavy ones are in flight.
+constexpr double kHeavyFlops = 1.5e12;
+
+// Algorithmic flops of a request: the disp_mmf! structure per right-hand
+// side (5 N log2 N (2M + 2M^2) + 2 M^4 N + 17 M^2 N + 16 M N), times the
+// fixed step count (4 per step) or, for adaptive single-mode solves, about
+// 100 accepted steps per metre (7 evaluations each), times 4.5 for a
+// forward + adjoint request.
+double request_flops(int nt, int m, int nsteps, double length_m, bool grad)
+{
+    const double N = nt, M = m;
+    const double rhs = 5 * N * std::log2(N) * (2 * M + 2 * M * M) + 2 * M * M * M * M * N + 17 * M * M * N + 16 * M * N;
+    const double evals = m > 1 ? 4.0 * nsteps : 7.0 * std::max(20.0, 100.0 * length_m);
+    return rhs * evals * (grad ? 4.5 : 1.0);
+}
 
 int mm_grad_cap(const fh_server_s* S) { return S->cfg.mm_grad_max_steps > 0 ? S->cfg.mm_grad_max_steps : 2000; }
 
@@ -156,20 +208,25 @@ void evict_for(fh_server_s* S, int64_t need)
             if (!it->busy && (victim == S->plans.end() || it->last_used < victim->last_used)) victim = it;
         if (victim == S->plans.end()) return;
         S->plan_bytes -= victim->bytes;
+        const int64_t t0 = now_ns();
         fh_plan_destroy(victim->plan);
+        trace_slow("plan release", t0, (int)(victim->bytes >> 20));
         S->plans.erase(victim);
     }
 }
 
 // An idle plan instance for `key`, created if needed.  Returns nullptr with
 // *wait = true when the budget is taken by running launches.
-PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
+// Called without the request lock: S->plans, S->plan_bytes and the lanes
+// belong to the scheduler thread, and `op` (an element of S->ops, which
+// submissions only insert into) stays valid.
+PlanInst* acquire_plan(fh_server_s* S, uint64_t key, OpSet* op, hipStream_t lane_stream, int* rc, bool* wait)
 {
     *rc = FH_OK;
     *wait = false;
     for (auto& pi : S->plans)
         if (pi.key == key && !pi.busy) return &pi;
-    const OpSet& o = S->ops[key];
+    const OpSet& o = *op;
     fh_problem_desc d{};
     d.nt = o.nt;
     d.modes = o.modes;
@@ -184,7 +241,6 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
         d.steps = cap;
         d.checkpoint_every = std::max(1, (int)std::lround(std::sqrt((double)cap * M / (4.0 * (M + qc)))));
     }
-    d.max_batch = S->cfg.max_batch;
     d.length_m = 1e-3;  // per-solve lengths override; the phase bound is checked per request
     d.dispersion = o.disp.data();
     d.gamma = o.gamma.data();
@@ -196,51 +252,109 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, int* rc, bool* wait)
     d.atol = S->cfg.atol;
     d.max_steps = S->cfg.max_steps;
     if (o.modes == 1 && o.grad) d.tape_segments = S->cfg.tape_segments;
-    fh_plan p = nullptr;
-    *rc = fh_plan_create(S->ctx, &d, &p);
-    if (*rc == FH_OK && o.grad) *rc = fhi::plan_prepare_gradient(p);
-    if (*rc != FH_OK) {
-        if (p) fh_plan_destroy(p);
-        if (*rc == FH_ENOMEM) {  // fragmentation or a full device: retry after running work drains
-            for (auto& l : S->lanes)
-                if (l.busy) {
-                    *wait = true;
-                    *rc = FH_OK;
-                }
+    if (op->capacity == 0) {
+        // one launch of a class holds at most a sixteenth of the memory
+        // budget, so plans of many classes stay resident together (releasing
+        // one synchronises the device): classes with large per-solve
+        // footprints (multimode gradients, FP64 at 32768 samples) get smaller
+        // batches instead of no admission
+        int64_t b1 = 0, b2 = 0;
+        d.max_batch = 1;
+        int crc = fh_plan_check(&d, o.grad, &b1);
+        d.max_batch = 2;
+        if (!crc) crc = fh_plan_check(&d, o.grad, &b2);
+        int cap = 1;
+        if (!crc && b2 > b1) {
+            const int64_t per = b2 - b1, fixed = b1 - per, share = S->cfg.memory_budget_bytes / 16;
+            cap = (int)std::max<int64_t>(1, std::min<int64_t>(S->cfg.max_batch, (share - fixed) / per));
+        } else if (!crc) {
+            cap = S->cfg.max_batch;
         }
-        return nullptr;
+        op->capacity = cap;
     }
-    const int64_t bytes = fhi::plan_bytes(p);
-    evict_for(S, bytes);
-    if (S->plan_bytes + bytes > S->cfg.memory_budget_bytes) {
-        fh_plan_destroy(p);
-        for (auto& l : S->lanes)
-            if (l.busy) {
-                *wait = true;
-                return nullptr;
-            }
+    d.max_batch = op->capacity;
+    int64_t bytes = 0;
+    if ((*rc = fh_plan_check(&d, o.grad, &bytes))) return nullptr;
+    if (bytes > S->cfg.memory_budget_bytes) {
         *rc = fhi::set_error(FH_ENOMEM, "one launch of this request class needs " + std::to_string(bytes) +
                                             " device bytes, above the server's memory budget");
         return nullptr;
     }
-    S->plans.push_back(PlanInst{p, key, false, now_ns(), bytes});
+    auto any_busy = [&] {
+        for (auto& l : S->lanes)
+            if (l.busy) return true;
+        return false;
+    };
+    evict_for(S, bytes);
+    if (S->plan_bytes + bytes > S->cfg.memory_budget_bytes) {
+        if (any_busy()) *wait = true;  // running launches hold the rest of the budget
+        else *rc = fhi::set_error(FH_ENOMEM, "the server's memory budget is held by plans");
+        return nullptr;
+    }
+    fh_plan p = nullptr;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const int64_t tp = now_ns();
+        *rc = fhi::plan_create_on(S->ctx, &d, lane_stream, &S->arena, &p);
+        if (*rc == FH_OK && o.grad) *rc = fhi::plan_prepare_gradient(p);
+        trace_slow("plan creation", tp, d.max_batch);
+        if (*rc == FH_OK) break;
+        if (p) fh_plan_destroy(p);
+        p = nullptr;
+        if (*rc != FH_ENOMEM) return nullptr;
+        // the arena is fragmented: release every idle plan and try once more
+        evict_for(S, S->cfg.memory_budget_bytes + 1);
+    }
+    if (!p) {
+        if (any_busy()) {  // fragmentation: retry after running work drains
+            *wait = true;
+            *rc = FH_OK;
+        }
+        return nullptr;
+    }
+    bytes = fhi::plan_bytes(p);
+    S->plans.push_back(PlanInst{p, key, d.max_batch, false, now_ns(), bytes});
     S->plan_bytes += bytes;
     return &S->plans.back();
 }
 
-// Start one batch on an idle lane.  Returns true if work was dispatched or
-// failed requests were finished, false when it has to wait.
-bool dispatch(fh_server_s* S, Lane& L)
+// Start one batch on an idle lane.  Entered and left with the request lock
+// held; released around the HIP calls.  Returns true if work was dispatched
+// or failed requests were finished, false when it has to wait.
+bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
 {
-    const uint64_t key = S->reqs[S->queue.front()].key;
+    // the oldest request that may run now: heavy requests only while fewer
+    // than lanes - lanes/4 lanes run heavy batches
+    int heavy_busy = 0;
+    for (const auto& l : S->lanes) heavy_busy += l.busy && l.heavy;
+    const int heavy_cap = std::max(1, (int)S->lanes.size() - (int)S->lanes.size() / 4);
+    const Req* first = nullptr;
+    for (int64_t id : S->queue) {
+        const Req& r = S->reqs[id];
+        if (!r.heavy || heavy_busy < heavy_cap) {
+            first = &r;
+            break;
+        }
+    }
+    if (!first) return false;
+    const uint64_t key = first->key;
+    const bool heavy = first->heavy;
+    OpSet* op = &S->ops.at(key);
     int rc;
     bool wait;
-    PlanInst* pi = acquire_plan(S, key, &rc, &wait);
+    lk.unlock();
+    const int64_t ta = now_ns();
+    PlanInst* pi = acquire_plan(S, key, op, L.s, &rc, &wait);
+    trace_slow("acquire_plan", ta, op->nt * op->modes);
+    lk.lock();
     if (wait) return false;
+    // only this thread removes from the queue: the class's requests are still there
     std::vector<int64_t> ids;
-    for (auto it = S->queue.begin(); it != S->queue.end() && (int)ids.size() < S->cfg.max_batch;) {
-        if (S->reqs[*it].key == key) {
+    std::vector<Req*> rq;
+    const int capacity = pi ? pi->capacity : S->cfg.max_batch;
+    for (auto it = S->queue.begin(); it != S->queue.end() && (int)ids.size() < capacity;) {
+        if (S->reqs[*it].key == key && S->reqs[*it].heavy == heavy) {
             ids.push_back(*it);
+            rq.push_back(&S->reqs[*it]);
             it = S->queue.erase(it);
         } else {
             ++it;
@@ -251,8 +365,17 @@ bool dispatch(fh_server_s* S, Lane& L)
         S->cv_done.notify_all();
         return true;
     }
-    const OpSet& o = S->ops[key];
+    const int64_t t0 = now_ns();
+    for (Req* r : rq) {
+        r->state = 1;
+        r->res.start_ns = t0;
+        r->res.batch = (int)ids.size();
+    }
+    // requests in state 1 are neither erased nor modified by other threads
+    lk.unlock();
+    const OpSet& o = *op;
     const int B = (int)ids.size(), nf = o.nt * o.modes;
+    const int64_t ts = now_ns();
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;
     const size_t es = fp64 ? sizeof(double) : sizeof(float);
     int e = FH_OK;
@@ -264,9 +387,8 @@ bool dispatch(fh_server_s* S, Lane& L)
     if (!e) e = L.h_status.ensure(B);
     if (!e) e = L.h_nst.ensure(B);
     if (!e) e = L.h_stats.ensure(3 * (size_t)B);
-    const int64_t t0 = now_ns();
     for (int b = 0; e == FH_OK && b < B; ++b) {
-        Req& r = S->reqs[ids[b]];
+        const Req& r = *rq[b];
         if (fp64) {
             std::memcpy(L.h_in.p + (size_t)b * nf * 2 * es, r.u0.data(), (size_t)nf * 2 * es);
         } else {
@@ -275,9 +397,6 @@ bool dispatch(fh_server_s* S, Lane& L)
         }
         L.h_len.p[b] = r.length;
         L.h_nst.p[b] = r.nsteps;
-        r.state = 1;
-        r.res.start_ns = t0;
-        r.res.batch = B;
     }
     if (e == FH_OK && hipMemcpyAsync(fhi::plan_u_in(pi->plan), L.h_in.p, (size_t)B * nf * 2 * es,
                                      hipMemcpyHostToDevice, L.s) != hipSuccess)
@@ -289,35 +408,46 @@ bool dispatch(fh_server_s* S, Lane& L)
         e = fhi::enqueue_batch(pi->plan, B, o.grad, L.d_len, o.modes > 1 ? L.d_nst : nullptr, L.s, L.h_out.p,
                                L.h_grad.p, L.h_cost.p, L.h_status.p, L.h_stats.p);
     if (e == FH_OK && hipEventRecord(L.ev, L.s) != hipSuccess) e = fhi::set_error(FH_EDEVICE, "hipEventRecord failed");
+    trace_slow("staging and launch", ts, B);
     if (e != FH_OK) {
         (void)hipStreamSynchronize(L.s);
-        for (int64_t id : ids) finish(S, id, e);
         pi->last_used = now_ns();
+        lk.lock();
+        for (int64_t id : ids) finish(S, id, e);
         S->cv_done.notify_all();
         return true;
     }
     pi->busy = true;
     L.busy = true;
+    L.heavy = heavy;
     L.pi = pi;
     L.ids = std::move(ids);
     L.grad = o.grad;
     L.modes = o.modes;
     L.nt = o.nt;
+    lk.lock();
     return true;
 }
 
-void complete(fh_server_s* S, Lane& L, int device_rc)
+// Scatter a finished lane's outputs to the callers' buffers (without the
+// request lock: the requests are in state 1) and finish them.  Entered and
+// left with the lock held.
+void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mutex>& lk)
 {
+    std::vector<Req*> rq;
+    for (int64_t id : L.ids) rq.push_back(&S->reqs[id]);
+    lk.unlock();
+    const int64_t tc = now_ns();
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;
     const size_t es = fp64 ? sizeof(double) : sizeof(float);
     const int nf = L.nt * L.modes;
     const bool adaptive = fhi::plan_adaptive(L.pi->plan);
-    for (size_t b = 0; b < L.ids.size(); ++b) {
-        Req& r = S->reqs[L.ids[b]];
-        int code = device_rc;
-        if (code == FH_OK) {
+    std::vector<int> codes(rq.size(), device_rc);
+    for (size_t b = 0; b < rq.size(); ++b) {
+        Req& r = *rq[b];
+        if (codes[b] == FH_OK) {
             const int st = L.h_status.p[b];
-            code = (st & 2) ? FH_EMAXSTEPS : (st ? FH_ENONFINITE : FH_OK);
+            codes[b] = (st & 2) ? FH_EMAXSTEPS : (st ? FH_ENONFINITE : FH_OK);
             if (fp64) {
                 std::memcpy(r.uL, L.h_out.p + b * nf * 2 * es, (size_t)nf * 2 * es);
             } else {
@@ -337,11 +467,13 @@ void complete(fh_server_s* S, Lane& L, int device_rc)
                 r.res.rhs_evals = 4LL * r.nsteps;
             }
         }
-        finish(S, L.ids[b], code);
     }
     L.pi->busy = false;
     L.pi->last_used = now_ns();
     L.pi = nullptr;
+    trace_slow("output scatter", tc, (int)rq.size());
+    lk.lock();
+    for (size_t b = 0; b < L.ids.size(); ++b) finish(S, L.ids[b], codes[b]);
     L.ids.clear();
     L.busy = false;
     S->cv_done.notify_all();
@@ -358,12 +490,12 @@ void worker_main(fh_server_s* S)
             const hipError_t q = hipEventQuery(L.ev);
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess) (void)hipStreamSynchronize(L.s);
-            complete(S, L, q == hipSuccess ? FH_OK : fhi::set_error(FH_EDEVICE, hipGetErrorString(q)));
+            complete(S, L, q == hipSuccess ? FH_OK : fhi::set_error(FH_EDEVICE, hipGetErrorString(q)), lk);
             progressed = true;
         }
         for (auto& L : S->lanes) {
             if (L.busy || S->queue.empty()) continue;
-            if (!dispatch(S, L)) break;
+            if (!dispatch(S, L, lk)) break;
             progressed = true;
         }
         bool any_busy = false;
@@ -416,7 +548,7 @@ extern "C" {
 int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out)
 {
     if (!ctx || !cfg || !out) return fhi::set_error(FH_EINVAL, "context, config or out is NULL");
-    if (cfg->lanes < 1 || cfg->lanes > 16) return fhi::set_error(FH_EINVAL, "lanes must lie in [1, 16]");
+    if (cfg->lanes < 1 || cfg->lanes > 32) return fhi::set_error(FH_EINVAL, "lanes must lie in [1, 32]");
     if (cfg->max_batch < 1) return fhi::set_error(FH_EINVAL, "max_batch must be positive");
     if (cfg->memory_budget_bytes <= 0) return fhi::set_error(FH_EINVAL, "memory budget must be positive");
     if (cfg->precision != FH_PREC_FP64 && cfg->precision != FH_PREC_FP32_BULK) return fhi::set_error(FH_EINVAL, "unknown precision");
@@ -427,6 +559,10 @@ int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out
     fh_server_s* S = new fh_server_s;
     S->ctx = ctx;
     S->cfg = *cfg;
+    if (int rc = S->arena.init((size_t)cfg->memory_budget_bytes)) {
+        delete S;
+        return rc;
+    }
     S->lanes.resize(cfg->lanes);
     for (auto& L : S->lanes) {
         // non-blocking: plan uploads on the null stream do not wait for running lanes
@@ -466,6 +602,7 @@ int fh_server_submit(fh_server S, const fh_request* r, int64_t* id)
     q.key = h.h;
     q.length = r->length_m;
     q.nsteps = r->modes > 1 ? std::max(16, (int)std::ceil(r->length_m / S->cfg.multimode_step_m)) : 0;
+    q.heavy = request_flops(r->nt, r->modes, q.nsteps, r->length_m, grad) > kHeavyFlops;
     q.u0.assign(r->u0, r->u0 + 2 * nf);
     q.uL = r->uL;
     q.grad = r->grad;
@@ -525,7 +662,11 @@ int fh_server_destroy(fh_server S)
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
     (void)hipSetDevice(S->ctx->device);
+    for (auto& L : S->lanes)
+        if (L.s) (void)hipStreamSynchronize(L.s);
     for (auto& pi : S->plans) fh_plan_destroy(pi.plan);
+    S->plans.clear();
+    S->arena.destroy();
     for (auto& L : S->lanes) {
         if (L.s) (void)hipStreamSynchronize(L.s);
         L.h_in.release();

@@ -41,11 +41,14 @@ int fail(int code, const std::string& msg)
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    fhi::Arena* arena = nullptr;  // owner of p when it came from an arena
 };
 
 struct fh_plan_s {
     fh_context ctx;
     fh_problem_desc d;
+    hipStream_t copy_stream = nullptr;  // stream of the table uploads at creation (null: the context stream)
+    fhi::Arena* arena = nullptr;        // source of the plan's device buffers (null: hipMalloc)
     int threads;  // workgroup size of the plan's kernels (single-mode or multimode shape)
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
@@ -86,6 +89,14 @@ namespace {
 int dev_alloc(fh_plan p, DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
+    if (p->arena) {
+        b.p = p->arena->alloc(bytes);
+        if (!b.p) return fail(FH_ENOMEM, "the server's device memory has no free range of " + std::to_string(bytes) + " bytes");
+        b.arena = p->arena;
+        b.bytes = bytes;
+        p->total_bytes += (int64_t)bytes;
+        return FH_OK;
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -99,9 +110,13 @@ int dev_alloc(fh_plan p, DevBuf& b, size_t bytes)
 
 void dev_free(DevBuf& b)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && b.arena)
+        b.arena->release(b.p, b.bytes);
+    else if (b.p)
+        (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+    b.arena = nullptr;
 }
 
 // Free a plan buffer and take it off the plan's byte count.
@@ -428,6 +443,10 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
 {
     const fh_problem_desc& d = p->d;
     const int nt = d.nt;
+    // uploads are asynchronous on one stream and synchronised before the host
+    // tables go out of scope: a synchronous copy runs on the null stream,
+    // whose hardware queue may be shared with a stream running a long launch
+    const hipStream_t cs = p->copy_stream ? p->copy_stream : p->ctx->stream;
     std::vector<std::complex<long double>> tw;
     int T = 0;
     if (!L.mm) {
@@ -468,16 +487,18 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
     if ((rc = dev_alloc(p, p->hn, sizeof(R) * hn.size()))) return rc;
     if ((rc = dev_alloc(p, p->shn, sizeof(R) * shn.size()))) return rc;
     if ((rc = dev_alloc(p, p->tw, sizeof(R) * twr.size()))) return rc;
-    FH_HIP(hipMemcpy(p->disp.p, d.dispersion, p->disp.bytes, hipMemcpyHostToDevice));
-    FH_HIP(hipMemcpy(p->hn.p, hn.data(), p->hn.bytes, hipMemcpyHostToDevice));
-    FH_HIP(hipMemcpy(p->shn.p, shn.data(), p->shn.bytes, hipMemcpyHostToDevice));
-    FH_HIP(hipMemcpy(p->tw.p, twr.data(), p->tw.bytes, hipMemcpyHostToDevice));
+    FH_HIP(hipMemcpyAsync(p->disp.p, d.dispersion, p->disp.bytes, hipMemcpyHostToDevice, cs));
+    FH_HIP(hipMemcpyAsync(p->hn.p, hn.data(), p->hn.bytes, hipMemcpyHostToDevice, cs));
+    FH_HIP(hipMemcpyAsync(p->shn.p, shn.data(), p->shn.bytes, hipMemcpyHostToDevice, cs));
+    FH_HIP(hipMemcpyAsync(p->tw.p, twr.data(), p->tw.bytes, hipMemcpyHostToDevice, cs));
+    FH_HIP(hipStreamSynchronize(cs));
     if (d.band_mask) {
         bool any = false;
         for (int k = 0; k < nt; ++k) any |= d.band_mask[k] != 0;
         if (!any) return fail(FH_EINVAL, "band mask must contain at least one bin");
         if ((rc = dev_alloc(p, p->mask, (size_t)nt))) return rc;
-        FH_HIP(hipMemcpy(p->mask.p, d.band_mask, (size_t)nt, hipMemcpyHostToDevice));
+        FH_HIP(hipMemcpyAsync(p->mask.p, d.band_mask, (size_t)nt, hipMemcpyHostToDevice, cs));
+        FH_HIP(hipStreamSynchronize(cs));
         p->has_mask = true;
     }
     p->every = L.every;
@@ -508,8 +529,9 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
             }
         if ((rc = dev_alloc(p, p->gr, sizeof(R) * gr.size()))) return rc;
         if ((rc = dev_alloc(p, p->qidx, sizeof(int) * L.qidx.size()))) return rc;
-        FH_HIP(hipMemcpy(p->gr.p, gr.data(), p->gr.bytes, hipMemcpyHostToDevice));
-        FH_HIP(hipMemcpy(p->qidx.p, L.qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice));
+        FH_HIP(hipMemcpyAsync(p->gr.p, gr.data(), p->gr.bytes, hipMemcpyHostToDevice, cs));
+        FH_HIP(hipMemcpyAsync(p->qidx.p, L.qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice, cs));
+        FH_HIP(hipStreamSynchronize(cs));
     }
     const size_t mb = (size_t)d.max_batch;
     if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
@@ -1189,7 +1211,77 @@ int fh_context_destroy(fh_context ctx)
     return FH_OK;
 }
 
-int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
+}  // extern "C"
+
+namespace fhi {
+
+namespace {
+constexpr size_t kGranule = 256;
+size_t granules(size_t bytes) { return (bytes + kGranule - 1) / kGranule * kGranule; }
+}  // namespace
+
+int Arena::init(size_t bytes)
+{
+    size = granules(bytes);
+    const hipError_t e = hipMalloc((void**)&base, size);
+    if (e != hipSuccess) {
+        base = nullptr;
+        size = 0;
+        return fail(e == hipErrorOutOfMemory ? FH_ENOMEM : FH_EDEVICE,
+                    "hipMalloc of the server's device memory (" + std::to_string(bytes) + " bytes): " +
+                        hipGetErrorString(e));
+    }
+    free_.clear();
+    free_[0] = size;
+    return FH_OK;
+}
+
+void Arena::destroy()
+{
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    size = 0;
+    free_.clear();
+}
+
+void* Arena::alloc(size_t bytes)
+{
+    const size_t n = granules(bytes);
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < n) continue;
+        const size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[off + n] = len - n;
+        return base + off;
+    }
+    return nullptr;
+}
+
+void Arena::release(void* p, size_t bytes)
+{
+    if (!p) return;
+    size_t off = (size_t)((char*)p - base), n = granules(bytes);
+    std::lock_guard<std::mutex> lk(mu);
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + n == next->first) {  // merge with the following free range
+        n += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {  // merge with the preceding free range
+            prev->second += n;
+            return;
+        }
+    }
+    free_[off] = n;
+}
+
+// fh_plan_create with the table uploads on `copy_stream` (null: the context
+// stream) and the device buffers from `arena` (null: hipMalloc).  The request
+// server passes the idle lane that will run the plan and its arena.
+int plan_create_on(fh_context ctx, const fh_problem_desc* desc, hipStream_t copy_stream, Arena* arena, fh_plan* out)
 {
     if (!ctx || !out) return fail(FH_EINVAL, "context or out is NULL");
     int rc = check_desc(desc);
@@ -1203,7 +1295,10 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     p->gamma0 = desc->gamma[0];
     p->adaptive = desc->scheme == FH_SCHEME_TSIT5;
     p->gradient_bytes = L.gradient_bytes;
+    p->copy_stream = copy_stream;
+    p->arena = arena;
     rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p, L) : upload_tables<float>(p, L);
+    p->copy_stream = nullptr;
     p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
     p->d.band_mask = nullptr;
     if (rc) {
@@ -1212,6 +1307,15 @@ int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
     }
     *out = p;
     return FH_OK;
+}
+
+}  // namespace fhi
+
+extern "C" {
+
+int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
+{
+    return fhi::plan_create_on(ctx, desc, nullptr, nullptr, out);
 }
 
 int fh_plan_check(const fh_problem_desc* desc, int32_t gradient, int64_t* bytes)

@@ -31,7 +31,7 @@ class Server:
     keep the stage tape of their first tape_segments segments of 50 accepted
     steps (no recomputation there; more device memory per plan)."""
 
-    def __init__(self, memory_budget_bytes: int = 64 << 30, lanes: int = 4, max_batch: int = 64,
+    def __init__(self, memory_budget_bytes: int = 16 << 30, lanes: int = 4, max_batch: int = 64,
                  precision: str = "fp32", rtol: float = 1e-6, atol: float = 1e-6, multimode_step_m: float = 1e-4,
                  max_steps: int = 10000, mm_grad_max_steps: int = 2000, context: Optional[Context] = None,
                  tape_segments: int = 0):
@@ -101,6 +101,7 @@ class Server:
             keep = self._pending.pop(int(rid))
         out = dict(retcode=res.retcode, accepted=res.accepted, rejected=res.rejected, rhs_evaluations=res.rhs_evals,
                    batch=res.batch, latency_ms=(res.finish_ns - res.submit_ns) / 1e6,
+                   submit_ns=res.submit_ns, start_ns=res.start_ns, finish_ns=res.finish_ns,
                    queue_ms=(res.start_ns - res.submit_ns) / 1e6 if res.start_ns else None,
                    uL=keep["uL"].T)
         if "grad" in keep:
