@@ -291,9 +291,9 @@ def serving_problem(cache: dict, nt: int, modes: int):
     return cache[key]
 
 
-def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, budget_gib=96):
+def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, budget_gib=96, drop_fields=False):
     """Submit the trace with Poisson arrivals at `rate` requests/s from a feeder
-    thread and wait for every request; returns per-request results."""
+    thread and collect every request as it finishes; returns per-request results."""
     import fiberhip
 
     rng = np.random.default_rng(7)
@@ -303,7 +303,9 @@ def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, 
         p, peak, basis, mask = serving_problem(cache, r["nt"], r["modes"])
         u0 = p.uω0 * np.sqrt(r["peak"] / peak) * np.exp(1j * (basis @ r["coef"]))[:, None]
         inputs.append((p, u0, mask))
+    print(f"bench: serving {precision}: {len(trace)} launches prepared", file=sys.stderr, flush=True)
     ids, rejected = [None] * len(trace), {}
+    submitted = [0]
     srv = fiberhip.Server(memory_budget_bytes=int(budget_gib * (1 << 30)), lanes=lanes, max_batch=max_batch,
                           precision=precision,
                           rtol=1e-6, atol=1e-6, multimode_step_m=5e-3, max_steps=20000)
@@ -323,25 +325,45 @@ def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, 
             except fiberhip.FiberHIPError as exc:
                 cls = f"M={r['modes']} {r['kind']} nt={r['nt']}" if exc.code == fiberhip._lib.FH_EUNSUPPORTED else str(exc)
                 rejected[cls] = rejected.get(cls, 0) + 1
+            submitted[0] = i + 1
+            if (i + 1) % max(1, len(trace) // 10) == 0:
+                print(f"bench: serving {precision}: {i + 1} of {len(trace)} requests submitted, "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     t_start = time.perf_counter()
     th = threading.Thread(target=feed)
     th.start()
-    th.join()
     results = [None] * len(trace)
-    done = 0
-    for i, rid in enumerate(ids):
-        if rid is None:
-            continue
-        try:
-            results[i] = srv.wait(rid)
-            results[i]["latency_ms"] = (results[i]["finish_ns"] - t0_ns) / 1e6 - arrivals[i] * 1e3
-        except fiberhip.FiberHIPError as exc:
-            results[i] = dict(error=str(exc))
-        done += 1
-        if done % max(1, len(trace) // 10) == 0:
-            print(f"bench: serving {precision}: {done} of {len(trace)} requests finished, "
-                  f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+    open_ = set()
+    collected, next_seen, last_print = 0, 0, time.perf_counter()
+    # collect in completion order: poll the submitted requests, wait on the finished ones
+    while collected < len(trace):
+        while next_seen < submitted[0]:
+            if ids[next_seen] is not None:
+                open_.add(next_seen)
+            else:
+                collected += 1  # rejected at submission
+            next_seen += 1
+        finished = [i for i in open_ if srv.poll(ids[i]) == 2]
+        for i in finished:
+            open_.discard(i)
+            try:
+                results[i] = srv.wait(ids[i])
+                results[i]["latency_ms"] = (results[i]["finish_ns"] - t0_ns) / 1e6 - arrivals[i] * 1e3
+                results[i]["energy"] = np.sum(np.abs(results[i]["uL"]) ** 2, axis=0)
+                if drop_fields:
+                    del results[i]["uL"]
+                    results[i].pop("grad", None)
+            except fiberhip.FiberHIPError as exc:
+                results[i] = dict(error=str(exc))
+            collected += 1
+            if collected % max(1, len(trace) // 10) == 0 or time.perf_counter() - last_print > 30:
+                last_print = time.perf_counter()
+                print(f"bench: serving {precision}: {collected} of {len(trace)} requests finished, "
+                      f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+        if not finished:
+            time.sleep(0.05)
+    th.join()
     wall = time.perf_counter() - t_start
     srv.close()
     return results, rejected, wall
@@ -485,6 +507,8 @@ def main() -> None:
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-budget-gib", type=float, default=96.0, help="device memory of the server's plans")
     ap.add_argument("--serve-precisions", default="fp32,fp64", help="server precisions to run, comma-separated")
+    ap.add_argument("--serve-save", default="", help="write per-request records (npz) of each precision to "
+                    "this path prefix; output fields are not kept in memory")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
@@ -713,10 +737,25 @@ def main() -> None:
         cache = {}
         runs = {}
         for precision in args.serve_precisions.split(","):
-            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache, budget_gib=args.serve_budget_gib)
+            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache, budget_gib=args.serve_budget_gib,
+                                              drop_fields=bool(args.serve_save))
+            if args.serve_save:
+                ok_ = [r is not None and "error" not in r for r in res]
+                energy = np.full((len(trace), 10), np.nan)
+                for i, r in enumerate(res):
+                    if ok_[i]:
+                        energy[i, :r["energy"].size] = r["energy"]
+                np.savez(f"{args.serve_save}_{precision}.npz", ok=np.array(ok_), energy=energy,
+                         latency_ms=np.array([r["latency_ms"] if o else np.nan for r, o in zip(res, ok_)]),
+                         batch=np.array([r["batch"] if o else 0 for r, o in zip(res, ok_)]),
+                         nt=np.array([r["nt"] for r in trace]), modes=np.array([r["modes"] for r in trace]),
+                         kind=np.array([r["kind"] for r in trace]), length=np.array([r["length"] for r in trace]),
+                         wall_s=wall, rate=args.serve_rate)
             ok = [r for r in res if r is not None and "error" not in r]
             lat = np.array([r["latency_ms"] for r in ok])
+            ngrad = sum(1 for r, t in zip(res, trace) if r is not None and "error" not in r and t["kind"] == "gradient")
             runs[precision] = dict(results=res, rejected=rejected, wall_s=wall, served=len(ok),
+                                   gradients_per_s=ngrad / wall, forwards_per_s=(len(ok) - ngrad) / wall,
                                    failed=sum(1 for r in res if r is not None and "error" in r),
                                    p50_ms=float(np.percentile(lat, 50)) if lat.size else None,
                                    p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
@@ -727,15 +766,17 @@ def main() -> None:
         errs = []
         for a, b in zip(runs.get("fp32", {}).get("results", []), runs.get("fp64", {}).get("results", [])):
             if a and b and "error" not in a and "error" not in b:
-                ea, eb = np.sum(np.abs(a["uL"]) ** 2, axis=0), np.sum(np.abs(b["uL"]) ** 2, axis=0)
-                errs.append(float(np.max(np.abs(ea / eb - 1))))
+                errs.append(float(np.max(np.abs(a["energy"] / b["energy"] - 1))))
         serving = {
             "workload": f"serving trace: {args.serve_requests} requests, Poisson arrivals at {args.serve_rate}/s, "
                         f"Nt in 2^11..2^15, M in {{1,3,6,10}} (SMF-28 / 50 um GRIN HG), L log-uniform 0.1-5 m, "
                         f"peak 1-20 kW, GDD/TOD/FOD phases, 50% forward / 50% forward+adjoint; single mode adaptive "
                         f"Tsit5 rtol=atol=1e-6, multimode fixed 5 mm RK4IP steps",
             **{f"{p}_{k}": runs[p][k] for p in runs
-               for k in ("served", "failed", "p50_ms", "p99_ms", "solves_per_s", "wall_s", "mean_batch", "rejected")},
+               for k in ("served", "failed", "p50_ms", "p99_ms", "solves_per_s", "gradients_per_s", "forwards_per_s",
+                         "wall_s", "mean_batch", "rejected")},
+            **({"fp32_over_fp64_gradients_per_s": runs["fp32"]["gradients_per_s"] / runs["fp64"]["gradients_per_s"]}
+               if "fp32" in runs and "fp64" in runs and runs["fp64"]["gradients_per_s"] > 0 else {}),
             "lanes": SERVE_LANES, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
             "memory_budget_gib": args.serve_budget_gib,
             "latency": "from each request's scheduled Poisson arrival to its completion",

@@ -187,8 +187,15 @@ template <int N> struct LdsSize { static constexpr int value = N + N / 16; };
 // solves of one launch (relative 1e-7 in lam0, varying from run to run),
 // although a dataflow check of the compiler's s_waitcnt placement finds no
 // missing wait; compiling with every wait forced to zero also removes the
-// difference.  Measured cost: within run-to-run noise.
-__device__ __forceinline__ void drain_memory() { __builtin_amdgcn_s_waitcnt(0); }
+// difference.  Measured cost: within run-to-run noise.  The wait intrinsic
+// orders the hardware only; the empty asm with a memory clobber also keeps
+// the compiler from moving loads and stores across the point.
+__device__ __forceinline__ void drain_memory()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0);
+    asm volatile("" ::: "memory");
+}
 
 // Largest complex (interleaved) exchange buffer a kernel uses before it
 // switches to the planar exchange.  FP64 kernels stay at or below 128 KiB:
