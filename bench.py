@@ -23,15 +23,17 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 
 # The serving leg runs one launch per lane concurrently, each on its own
-# stream; HIP maps streams onto hardware queues per process (4 by default,
+# stream.  HIP maps streams onto hardware queues per process (4 by default,
 # and environments may pin that value), and work of streams sharing a queue
 # runs one after another: a short request on a free lane waited 4.5 s behind
-# a long launch with 4 queues and 4 lanes, and 1 ms with 20 queues and 16
-# lanes.  Four queues beyond the lanes keep the null and context streams
-# apart too.  HIP reads the value once, when it initialises, so it is set
-# here, before any HIP call, overriding an inherited value.
+# a long launch with 4 queues and 4 lanes, and 1-2 ms with 20 queues and 16
+# lanes.  The four extra queues keep the null and context streams apart.
+# Many more active queues oversubscribe the hardware scheduler: with 28
+# lanes on 32 queues a plan's table upload waited up to 160 s behind running
+# launches.  HIP reads the value once, when it initialises, so main() sets it
+# before any HIP call, overriding an inherited value.
 SERVE_LANES = 16
-os.environ["GPU_MAX_HW_QUEUES"] = str(SERVE_LANES + 4)
+HW_QUEUES = SERVE_LANES + 4
 
 import numpy as np  # noqa: E402
 
@@ -291,7 +293,7 @@ def serving_problem(cache: dict, nt: int, modes: int):
     return cache[key]
 
 
-def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, budget_gib=96, drop_fields=False):
+def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, budget_gib=160, drop_fields=False):
     """Submit the trace with Poisson arrivals at `rate` requests/s from a feeder
     thread and collect every request as it finishes; returns per-request results."""
     import fiberhip
@@ -485,6 +487,39 @@ def cpu_baseline(problem, steps, runs=5):
                 **_host_description())
 
 
+def serving_report(prefix: str) -> dict:
+    """Latency, throughput and FP32-vs-FP64 matched error of a serving trace
+    from the per-request records of both precisions."""
+    recs = {p: np.load(f"{prefix}_{p}.npz") for p in ("fp32", "fp64")}
+    out = {}
+    for p, r in recs.items():
+        ok = r["ok"]
+        lat = r["latency_ms"][ok]
+        grad = ok & (r["kind"] == "gradient")
+        wall = float(r["wall_s"])
+        out[p] = {"served": int(ok.sum()), "wall_s": wall, "solves_per_s": float(ok.sum() / wall),
+                  "gradients_per_s": float(grad.sum() / wall), "forwards_per_s": float((ok & ~grad).sum() / wall),
+                  "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+                  "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(np.mean(r["batch"][ok])),
+                  "rate_per_s": float(r["rate"])}
+    a, b = recs["fp32"], recs["fp64"]
+    both = a["ok"] & b["ok"]
+    err = np.nanmax(np.abs(a["energy"][both] / b["energy"][both] - 1), axis=1)
+    out["matched_requests"] = int(both.sum())
+    out["fp32_vs_fp64_rel_mode_energy_err"] = {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
+                                               "within_1e-5": int((err <= 1e-5).sum())}
+    out["fp32_over_fp64_gradients_per_s"] = out["fp32"]["gradients_per_s"] / out["fp64"]["gradients_per_s"]
+    out["fp32_over_fp64_solves_per_s"] = out["fp32"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    by = {}
+    for m in (1, 3, 6, 10):
+        sel = a["ok"] & (a["modes"] == m)
+        if sel.any():
+            by[f"M={m}"] = {"fp32_p50_ms": float(np.percentile(a["latency_ms"][sel], 50)),
+                            "fp32_p99_ms": float(np.percentile(a["latency_ms"][sel], 99))}
+    out["fp32_latency_by_modes"] = by
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -505,14 +540,20 @@ def main() -> None:
                          "every segment)")
     ap.add_argument("--serve-requests", type=int, default=100, help="requests in the serving trace")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-budget-gib", type=float, default=96.0, help="device memory of the server's plans")
+    ap.add_argument("--serve-budget-gib", type=float, default=160.0, help="device memory of the server's plans")
     ap.add_argument("--serve-precisions", default="fp32,fp64", help="server precisions to run, comma-separated")
     ap.add_argument("--serve-save", default="", help="write per-request records (npz) of each precision to "
                     "this path prefix; output fields are not kept in memory")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
+    ap.add_argument("--serve-report", default="",
+                    help="summarise the records PREFIX_fp32.npz and PREFIX_fp64.npz written by --serve-save (no GPU)")
     args = ap.parse_args()
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+    if args.serve_report:
+        print(json.dumps(serving_report(args.serve_report)), flush=True)
+        return
     if args.copy_calibration_mib:
         import ctypes
 
@@ -640,6 +681,13 @@ def main() -> None:
         e32, e64 = res["fp32"]["energies"], res["fp64"]["energies"]
         per_mode = np.max(np.abs(e32 - e64) / e64, axis=0)
         flops = forward_flops(nt, m, steps)
+        # Kerr contraction on the matrix cores (FP32, M >= 6): per right-hand side
+        # and 32-sample chunk, ceil(Q/32) output tiles of ceil(P/2)
+        # v_mfma_f32_32x32x2f32 (4096 flops each), Q = P = M(M+1)/2 pairs; the
+        # useful part is the 2 Q P flops per sample of the reduced product
+        q = m * (m + 1) // 2
+        mfma_inst = 4 * steps * (nt // 32) * (-(-q // 32)) * (-(-q // 2))
+        mfma_rate = args.mm_batch * mfma_inst * 4096 / (res["fp32"]["kernel_ms"] / 1e3) / 1e12
         mm = {
             "workload": f"GRIN 50 um / NA 0.2, 0.3 m, M=6 HG modes (256x256), Nt={nt}, 12 ps, 10 nJ sech2 "
                         f"100 fs, seeded Dirichlet launches, {steps} RK4IP steps",
@@ -652,6 +700,11 @@ def main() -> None:
             "fp32_roofline_frac": args.mm_batch * flops / (res["fp32"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp32"],
             "fp64_roofline_frac": args.mm_batch * flops / (res["fp64"]["kernel_ms"] / 1e3) / 1e12 / PEAK_TFLOPS["fp64"],
             "flops_per_solve": flops,
+            "skc_mfma": {"instructions_per_solve": mfma_inst, "issued_tflops": mfma_rate,
+                         "frac_of_fp32_matrix_peak": mfma_rate / PEAK_TFLOPS["fp32"],
+                         "useful_fraction_of_issued": 2 * q * q * 32 / (4096 * (-(-q // 32)) * (-(-q // 2))),
+                         "source": "v_mfma_f32_32x32x2f32 count per solve, equal to SQ_INSTS_VALU_MFMA_F32 in "
+                                   "profiles/k_rk4ip_multi_grin6_4096.md"},
         }
         if args.only == "multimode":
             line = {"metric": "fwd solves/sec (FP32-bulk batched path, GRIN M=6)", "value": mm["fp32_solves_per_s"],

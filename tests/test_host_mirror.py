@@ -401,3 +401,35 @@ def test_central_fd_weights():
     assert np.allclose(grin.central_fd_weights(5, 1), [1 / 12, -2 / 3, 0, 2 / 3, -1 / 12], atol=1e-15)
     assert np.allclose(grin.central_fd_weights(5, 2), [-1 / 12, 4 / 3, -5 / 2, 4 / 3, -1 / 12], atol=1e-14)
     assert np.allclose(grin.central_fd_weights(3, 1), [-0.5, 0, 0.5], atol=1e-15)
+
+
+def test_bench_serving_report(tmp_path):
+    """bench.py --serve-report combines the per-request records of both
+    precisions: latency percentiles, rates, and the FP32-vs-FP64 mode-energy
+    error over the requests both served."""
+    import importlib.util
+    import pathlib
+
+    path = pathlib.Path(__file__).resolve().parents[1] / "bench.py"
+    spec = importlib.util.spec_from_file_location("bench_for_test", str(path))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    n = 8
+    e = np.ones((n, 10)) * np.nan
+    e[:, :3] = 2.0
+    kind = np.array(["forward", "gradient"] * 4)
+    common = dict(nt=np.full(n, 1024), modes=np.array([1, 3, 6, 10] * 2), kind=kind, length=np.ones(n), rate=5.0)
+    np.savez(tmp_path / "t_fp64.npz", ok=np.ones(n, bool), energy=e, latency_ms=np.arange(n, dtype=float) * 10,
+             batch=np.ones(n), wall_s=4.0, **common)
+    e32 = e.copy()
+    e32[3, 1] = 2.0 * (1 + 3e-6)
+    ok32 = np.ones(n, bool)
+    ok32[5] = False
+    np.savez(tmp_path / "t_fp32.npz", ok=ok32, energy=e32, latency_ms=np.arange(n, dtype=float), batch=np.ones(n) * 2,
+             wall_s=1.0, **common)
+    r = bench.serving_report(str(tmp_path / "t"))
+    assert r["matched_requests"] == 7 and r["fp32"]["served"] == 7
+    assert abs(r["fp32_vs_fp64_rel_mode_energy_err"]["max"] - 3e-6) < 1e-12
+    assert r["fp32_vs_fp64_rel_mode_energy_err"]["within_1e-5"] == 7
+    assert r["fp32"]["gradients_per_s"] == 3.0 and r["fp64"]["gradients_per_s"] == 1.0
+    assert r["fp32_over_fp64_gradients_per_s"] == 3.0
