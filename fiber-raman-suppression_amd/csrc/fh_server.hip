@@ -160,6 +160,7 @@ struct fh_server_s {
     std::list<PlanInst> plans;
     std::vector<Lane> lanes;
     fhi::Arena arena;  // the memory budget, allocated once; plans sub-allocate from it
+    fhi::Arena stage;  // pinned host staging of plan tables, so plan creation does not wait for uploads
     int64_t next_id = 1, plan_bytes = 0;
     bool stop = false;
     std::thread worker;
@@ -175,6 +176,10 @@ constexpr int kMaxMultimodeSteps = 1 << 22;
 // may occupy at most lanes - lanes/4 lanes, so light requests keep lanes to
 // run on while heavy ones are in flight.
 constexpr double kHeavyFlops = 1.5e12;
+
+// Pinned host staging of plan tables (a plan of 32768 samples and 10 modes
+// stages about 4 MB); when it is full, uploads fall back to synchronous ones.
+constexpr size_t kStageBytes = 256ull << 20;
 
 // Algorithmic flops of a request: the disp_mmf! structure per right-hand
 // side (5 N log2 N (2M + 2M^2) + 2 M^4 N + 17 M^2 N + 16 M N), times the
@@ -294,7 +299,7 @@ PlanInst* acquire_plan(fh_server_s* S, uint64_t key, OpSet* op, hipStream_t lane
     fh_plan p = nullptr;
     for (int attempt = 0; attempt < 2; ++attempt) {
         const int64_t tp = now_ns();
-        *rc = fhi::plan_create_on(S->ctx, &d, lane_stream, &S->arena, &p);
+        *rc = fhi::plan_create_on(S->ctx, &d, lane_stream, &S->arena, &S->stage, &p);
         if (*rc == FH_OK && o.grad) *rc = fhi::plan_prepare_gradient(p);
         trace_slow("plan creation", tp, d.max_batch);
         if (*rc == FH_OK) break;
@@ -563,6 +568,11 @@ int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out
         delete S;
         return rc;
     }
+    if (int rc = S->stage.init(kStageBytes, true)) {
+        S->arena.destroy();
+        delete S;
+        return rc;
+    }
     S->lanes.resize(cfg->lanes);
     for (auto& L : S->lanes) {
         // non-blocking: plan uploads on the null stream do not wait for running lanes
@@ -667,6 +677,7 @@ int fh_server_destroy(fh_server S)
     for (auto& pi : S->plans) fh_plan_destroy(pi.plan);
     S->plans.clear();
     S->arena.destroy();
+    S->stage.destroy();
     for (auto& L : S->lanes) {
         if (L.s) (void)hipStreamSynchronize(L.s);
         L.h_in.release();

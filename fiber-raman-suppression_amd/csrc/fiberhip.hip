@@ -49,6 +49,8 @@ struct fh_plan_s {
     fh_problem_desc d;
     hipStream_t copy_stream = nullptr;  // stream of the table uploads at creation (null: the context stream)
     fhi::Arena* arena = nullptr;        // source of the plan's device buffers (null: hipMalloc)
+    fhi::Arena* stage = nullptr;        // pinned host staging of the table uploads (null: synchronous uploads)
+    std::vector<std::pair<void*, size_t>> staged;  // staging chunks held until the plan is destroyed
     int threads;  // workgroup size of the plan's kernels (single-mode or multimode shape)
     DevBuf disp, hn, shn, tw, work, status, u_in, u_out;
     DevBuf mask, ckpt, tape_u, tape_h, g_lam0, g_grad, g_cost, g_seed;  // adjoint, allocated on first use
@@ -447,6 +449,21 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
     // tables go out of scope: a synchronous copy runs on the null stream,
     // whose hardware queue may be shared with a stream running a long launch
     const hipStream_t cs = p->copy_stream ? p->copy_stream : p->ctx->stream;
+    // one table upload: through the plan's pinned staging, left in flight;
+    // otherwise from the caller's memory, synchronised before returning
+    auto upload = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (p->stage && p->copy_stream) {
+            if (void* st = p->stage->alloc(bytes)) {
+                std::memcpy(st, src, bytes);
+                p->staged.emplace_back(st, bytes);
+                FH_HIP(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, cs));
+                return FH_OK;
+            }
+        }
+        FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cs));
+        FH_HIP(hipStreamSynchronize(cs));
+        return FH_OK;
+    };
     std::vector<std::complex<long double>> tw;
     int T = 0;
     if (!L.mm) {
@@ -487,18 +504,16 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
     if ((rc = dev_alloc(p, p->hn, sizeof(R) * hn.size()))) return rc;
     if ((rc = dev_alloc(p, p->shn, sizeof(R) * shn.size()))) return rc;
     if ((rc = dev_alloc(p, p->tw, sizeof(R) * twr.size()))) return rc;
-    FH_HIP(hipMemcpyAsync(p->disp.p, d.dispersion, p->disp.bytes, hipMemcpyHostToDevice, cs));
-    FH_HIP(hipMemcpyAsync(p->hn.p, hn.data(), p->hn.bytes, hipMemcpyHostToDevice, cs));
-    FH_HIP(hipMemcpyAsync(p->shn.p, shn.data(), p->shn.bytes, hipMemcpyHostToDevice, cs));
-    FH_HIP(hipMemcpyAsync(p->tw.p, twr.data(), p->tw.bytes, hipMemcpyHostToDevice, cs));
-    FH_HIP(hipStreamSynchronize(cs));
+    if ((rc = upload(p->disp.p, d.dispersion, p->disp.bytes))) return rc;
+    if ((rc = upload(p->hn.p, hn.data(), p->hn.bytes))) return rc;
+    if ((rc = upload(p->shn.p, shn.data(), p->shn.bytes))) return rc;
+    if ((rc = upload(p->tw.p, twr.data(), p->tw.bytes))) return rc;
     if (d.band_mask) {
         bool any = false;
         for (int k = 0; k < nt; ++k) any |= d.band_mask[k] != 0;
         if (!any) return fail(FH_EINVAL, "band mask must contain at least one bin");
         if ((rc = dev_alloc(p, p->mask, (size_t)nt))) return rc;
-        FH_HIP(hipMemcpyAsync(p->mask.p, d.band_mask, (size_t)nt, hipMemcpyHostToDevice, cs));
-        FH_HIP(hipStreamSynchronize(cs));
+        if ((rc = upload(p->mask.p, d.band_mask, (size_t)nt))) return rc;
         p->has_mask = true;
     }
     p->every = L.every;
@@ -529,9 +544,8 @@ template <typename R> int upload_tables(fh_plan p, const PlanLayout& L)
             }
         if ((rc = dev_alloc(p, p->gr, sizeof(R) * gr.size()))) return rc;
         if ((rc = dev_alloc(p, p->qidx, sizeof(int) * L.qidx.size()))) return rc;
-        FH_HIP(hipMemcpyAsync(p->gr.p, gr.data(), p->gr.bytes, hipMemcpyHostToDevice, cs));
-        FH_HIP(hipMemcpyAsync(p->qidx.p, L.qidx.data(), p->qidx.bytes, hipMemcpyHostToDevice, cs));
-        FH_HIP(hipStreamSynchronize(cs));
+        if ((rc = upload(p->gr.p, gr.data(), p->gr.bytes))) return rc;
+        if ((rc = upload(p->qidx.p, L.qidx.data(), p->qidx.bytes))) return rc;
     }
     const size_t mb = (size_t)d.max_batch;
     if ((rc = dev_alloc(p, p->work, 2 * sizeof(R) * p->work_per_solve * mb))) return rc;
@@ -1220,10 +1234,11 @@ constexpr size_t kGranule = 256;
 size_t granules(size_t bytes) { return (bytes + kGranule - 1) / kGranule * kGranule; }
 }  // namespace
 
-int Arena::init(size_t bytes)
+int Arena::init(size_t bytes, bool pinned_host)
 {
     size = granules(bytes);
-    const hipError_t e = hipMalloc((void**)&base, size);
+    host = pinned_host;
+    const hipError_t e = host ? hipHostMalloc((void**)&base, size, hipHostMallocDefault) : hipMalloc((void**)&base, size);
     if (e != hipSuccess) {
         base = nullptr;
         size = 0;
@@ -1238,7 +1253,7 @@ int Arena::init(size_t bytes)
 
 void Arena::destroy()
 {
-    if (base) (void)hipFree(base);
+    if (base) (void)(host ? hipHostFree(base) : hipFree(base));
     base = nullptr;
     size = 0;
     free_.clear();
@@ -1281,7 +1296,8 @@ void Arena::release(void* p, size_t bytes)
 // fh_plan_create with the table uploads on `copy_stream` (null: the context
 // stream) and the device buffers from `arena` (null: hipMalloc).  The request
 // server passes the idle lane that will run the plan and its arena.
-int plan_create_on(fh_context ctx, const fh_problem_desc* desc, hipStream_t copy_stream, Arena* arena, fh_plan* out)
+int plan_create_on(fh_context ctx, const fh_problem_desc* desc, hipStream_t copy_stream, Arena* arena, Arena* stage,
+                   fh_plan* out)
 {
     if (!ctx || !out) return fail(FH_EINVAL, "context or out is NULL");
     int rc = check_desc(desc);
@@ -1297,7 +1313,9 @@ int plan_create_on(fh_context ctx, const fh_problem_desc* desc, hipStream_t copy
     p->gradient_bytes = L.gradient_bytes;
     p->copy_stream = copy_stream;
     p->arena = arena;
+    p->stage = stage;
     rc = desc->precision == FH_PREC_FP64 ? upload_tables<double>(p, L) : upload_tables<float>(p, L);
+    if (rc && !p->staged.empty()) (void)hipStreamSynchronize(copy_stream);  // no upload left in flight
     p->copy_stream = nullptr;
     p->d.dispersion = p->d.gamma = p->d.raman = p->d.shock = nullptr;
     p->d.band_mask = nullptr;
@@ -1315,7 +1333,7 @@ extern "C" {
 
 int fh_plan_create(fh_context ctx, const fh_problem_desc* desc, fh_plan* out)
 {
-    return fhi::plan_create_on(ctx, desc, nullptr, nullptr, out);
+    return fhi::plan_create_on(ctx, desc, nullptr, nullptr, nullptr, out);
 }
 
 int fh_plan_check(const fh_problem_desc* desc, int32_t gradient, int64_t* bytes)
@@ -1338,6 +1356,9 @@ int fh_plan_destroy(fh_plan p)
     for (DevBuf* b : {&p->disp, &p->hn, &p->shn, &p->tw, &p->work, &p->status, &p->u_in, &p->u_out, &p->mask,
                       &p->ckpt, &p->tape_u, &p->tape_h, &p->g_lam0, &p->g_grad, &p->g_cost, &p->g_seed, &p->gr, &p->qidx, &p->hs, &p->zs, &p->sstats})
         dev_free(*b);
+    // staged uploads completed before any launch that followed them on the
+    // copy stream; a plan is released only after its launches completed
+    for (auto& c : p->staged) p->stage->release(c.first, c.second);
     delete p;
     return FH_OK;
 }
