@@ -503,6 +503,9 @@ def serving_report(prefix: str) -> dict:
                   "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(np.mean(r["batch"][ok])),
                   "rate_per_s": float(r["rate"])}
     a, b = recs["fp32"], recs["fp64"]
+    n = min(a["ok"].size, b["ok"].size)  # one precision may have served a prefix of the trace
+    a = {k: (a[k][:n] if a[k].ndim else a[k]) for k in a.files}
+    b = {k: (b[k][:n] if b[k].ndim else b[k]) for k in b.files}
     both = a["ok"] & b["ok"]
     err = np.nanmax(np.abs(a["energy"][both] / b["energy"][both] - 1), axis=1)
     out["matched_requests"] = int(both.sum())
@@ -510,6 +513,8 @@ def serving_report(prefix: str) -> dict:
                                                "within_1e-5": int((err <= 1e-5).sum())}
     out["fp32_over_fp64_gradients_per_s"] = out["fp32"]["gradients_per_s"] / out["fp64"]["gradients_per_s"]
     out["fp32_over_fp64_solves_per_s"] = out["fp32"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["note"] = (f"fp32 served {out['fp32']['served']} and fp64 {out['fp64']['served']} requests of the same trace "
+                   f"(same arrivals); the error compares the {n} requests both were given")
     by = {}
     for m in (1, 3, 6, 10):
         sel = a["ok"] & (a["modes"] == m)
@@ -542,6 +547,8 @@ def main() -> None:
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-budget-gib", type=float, default=160.0, help="device memory of the server's plans")
     ap.add_argument("--serve-precisions", default="fp32,fp64", help="server precisions to run, comma-separated")
+    ap.add_argument("--serve-first", type=int, default=0,
+                    help="serve only the first K requests of the --serve-requests trace (same requests and arrivals)")
     ap.add_argument("--serve-save", default="", help="write per-request records (npz) of each precision to "
                     "this path prefix; output fields are not kept in memory")
     ap.add_argument("--mm-batch", type=int, default=256, help="solves per launch (GRIN M=6 forward workload)")
@@ -787,6 +794,8 @@ def main() -> None:
             line["gain_ydfa_4096"] = gain_line
     if args.only in ("all", "serving"):
         trace = serving_trace(args.serve_requests)
+        if args.serve_first:
+            trace = trace[:args.serve_first]
         cache = {}
         runs = {}
         for precision in args.serve_precisions.split(","):

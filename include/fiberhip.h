@@ -352,16 +352,24 @@ int fh_step_sizes(fh_plan plan, int32_t batch, double *hs);
  * their own length; multimode requests run the fixed-step RK4IP with
  * ceil(length / multimode_step_m) steps (at least 16); multimode gradients
  * are offered up to mm_grad_max_steps steps (their checkpoint capacity) for
- * fully permutation-symmetric gamma.  No counterpart exists in the reference (its
- * solves run one at a time in the calling Julia task); callers are batch
- * optimisation drivers and services that issue many solves. */
+ * fully permutation-symmetric gamma.  The budget is allocated once, at
+ * creation, as one device arena the plans sub-allocate from, beside a
+ * 256 MiB pinned host arena that stages plan tables.  Requests whose
+ * estimated work exceeds about 1.5e12 flops occupy at most three quarters
+ * of the lanes.  Lanes run on their own streams: give the process lanes + 4
+ * hardware queues (GPU_MAX_HW_QUEUES, read when HIP initialises) within the
+ * device's compute queues, or lanes sharing a queue run one after another.
+ * No counterpart exists in the reference (its solves run one at a time in
+ * the calling Julia task); callers are batch optimisation drivers and
+ * services that issue many solves. */
 
 typedef struct fh_server_s *fh_server;
 
 typedef struct {
     int64_t memory_budget_bytes; /* device memory the server's plans may hold      */
-    int32_t lanes;               /* concurrent launches (streams), >= 1            */
-    int32_t max_batch;           /* solves per launch                              */
+    int32_t lanes;               /* concurrent launches (streams), 1..32           */
+    int32_t max_batch;           /* solves per launch (at most; a class whose launch
+                                  * would exceed a sixteenth of the budget gets fewer) */
     int32_t precision;           /* FH_PREC_* of every request                     */
     int32_t max_steps;           /* adaptive accepted-step limit (0 = 10000)       */
     double rtol, atol;           /* adaptive tolerances of single-mode requests    */
