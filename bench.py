@@ -357,7 +357,7 @@ def run_serving(trace, precision, rate, cache, lanes=SERVE_LANES, max_batch=64, 
                     del results[i]["uL"]
                     results[i].pop("grad", None)
             except fiberhip.FiberHIPError as exc:
-                results[i] = dict(error=str(exc))
+                results[i] = dict(error=str(exc), code=exc.code)
             collected += 1
             if collected % max(1, len(trace) // 10) == 0 or time.perf_counter() - last_print > 30:
                 last_print = time.perf_counter()
@@ -516,12 +516,14 @@ def serving_report(prefix: str) -> dict:
     out["note"] = (f"fp32 served {out['fp32']['served']} and fp64 {out['fp64']['served']} requests of the same trace "
                    f"(same arrivals); the error compares the {n} requests both were given")
     by = {}
+    full = recs["fp32"]
     for m in (1, 3, 6, 10):
-        sel = a["ok"] & (a["modes"] == m)
-        if sel.any():
-            by[f"M={m}"] = {"fp32_p50_ms": float(np.percentile(a["latency_ms"][sel], 50)),
-                            "fp32_p99_ms": float(np.percentile(a["latency_ms"][sel], 99))}
-    out["fp32_latency_by_modes"] = by
+        for kind in ("forward", "gradient"):
+            sel = full["ok"] & (full["modes"] == m) & (full["kind"] == kind)
+            if sel.any():
+                by[f"M={m} {kind}"] = {"fp32_p50_ms": float(np.percentile(full["latency_ms"][sel], 50)),
+                                       "fp32_p99_ms": float(np.percentile(full["latency_ms"][sel], 99))}
+    out["fp32_latency_by_class"] = by
     return out
 
 
@@ -808,6 +810,8 @@ def main() -> None:
                     if ok_[i]:
                         energy[i, :r["energy"].size] = r["energy"]
                 np.savez(f"{args.serve_save}_{precision}.npz", ok=np.array(ok_), energy=energy,
+                         code=np.array([r.get("code", -1) if r is not None and "error" in r else 0 for r in res]),
+                         error=np.array([r["error"] if r is not None and "error" in r else "" for r in res]),
                          latency_ms=np.array([r["latency_ms"] if o else np.nan for r, o in zip(res, ok_)]),
                          batch=np.array([r["batch"] if o else 0 for r, o in zip(res, ok_)]),
                          nt=np.array([r["nt"] for r in trace]), modes=np.array([r["modes"] for r in trace]),
@@ -847,11 +851,12 @@ def main() -> None:
             "fp32_vs_fp64_p99_rel_mode_energy_err": float(np.percentile(errs, 99)) if errs else None,
         }
         if args.only == "serving":
-            line = {"metric": "serving solves/sec with p50/p99 latency (FP32-bulk)", "value": serving["fp32_solves_per_s"],
-                    "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-                    "ms_per_step": serving["fp32_wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
-                    "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": serving["workload"]},
-                    "serving": serving}
+            lead = "fp32" if "fp32" in runs else next(iter(runs))
+            line = {"metric": f"serving solves/sec with p50/p99 latency ({'FP32-bulk' if lead == 'fp32' else 'FP64'})",
+                    "value": serving[f"{lead}_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
+                    "warmup": 0, "ms_per_step": serving[f"{lead}_wall_s"] * 1e3, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None, "dtype": "f32" if lead == "fp32" else "f64",
+                    "data": "synthetic", "config": {"workload": serving["workload"]}, "serving": serving}
         else:
             line["serving_trace"] = serving
     print(json.dumps(line), flush=True)

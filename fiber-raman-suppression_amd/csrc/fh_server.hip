@@ -72,6 +72,7 @@ struct Req {
     std::vector<double> u0;
     double *uL = nullptr, *grad = nullptr, *cost = nullptr;
     int state = 0;  // 0 queued, 1 running, 2 finished
+    std::string msg;  // failure message
     fh_request_result res{};
 };
 
@@ -196,12 +197,14 @@ double request_flops(int nt, int m, int nsteps, double length_m, bool grad)
 
 int mm_grad_cap(const fh_server_s* S) { return S->cfg.mm_grad_max_steps > 0 ? S->cfg.mm_grad_max_steps : 2000; }
 
-void finish(fh_server_s* S, int64_t id, int code)
+// Finish a request; a failure keeps its message for the caller's fh_server_wait.
+void finish(fh_server_s* S, int64_t id, int code, const std::string& msg = std::string())
 {
     Req& r = S->reqs[id];
     r.res.retcode = code;
     r.res.finish_ns = now_ns();
     r.state = 2;
+    if (code) r.msg = msg;
 }
 
 // Release idle plans, least recently used first, until `need` more bytes fit.
@@ -366,7 +369,8 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
         }
     }
     if (!pi) {
-        for (int64_t id : ids) finish(S, id, rc);
+        const std::string m = fh_last_error();
+        for (int64_t id : ids) finish(S, id, rc, "class setup: " + m);
         S->cv_done.notify_all();
         return true;
     }
@@ -418,7 +422,8 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
         (void)hipStreamSynchronize(L.s);
         pi->last_used = now_ns();
         lk.lock();
-        for (int64_t id : ids) finish(S, id, e);
+        const std::string m = fh_last_error();
+        for (int64_t id : ids) finish(S, id, e, "launch: " + m);
         S->cv_done.notify_all();
         return true;
     }
@@ -478,7 +483,11 @@ void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mute
     L.pi = nullptr;
     trace_slow("output scatter", tc, (int)rq.size());
     lk.lock();
-    for (size_t b = 0; b < L.ids.size(); ++b) finish(S, L.ids[b], codes[b]);
+    for (size_t b = 0; b < L.ids.size(); ++b)
+        finish(S, L.ids[b], codes[b],
+               codes[b] == FH_EMAXSTEPS ? "adaptive solve reached max_steps"
+                                        : (codes[b] == FH_ENONFINITE ? "solve produced non-finite fields"
+                                                                     : "device error: " + std::string(fh_last_error())));
     L.ids.clear();
     L.busy = false;
     S->cv_done.notify_all();
@@ -657,8 +666,9 @@ int fh_server_wait(fh_server S, int64_t id, fh_request_result* res)
     if (!S->reqs.count(id)) return fhi::set_error(FH_EINVAL, "unknown request id");
     S->cv_done.wait(lk, [&] { return S->reqs[id].state == 2; });
     *res = S->reqs[id].res;
+    const std::string msg = S->reqs[id].msg;
     S->reqs.erase(id);
-    if (res->retcode) return fhi::set_error(res->retcode, "request " + std::to_string(id) + " failed");
+    if (res->retcode) return fhi::set_error(res->retcode, "request " + std::to_string(id) + " failed: " + msg);
     return FH_OK;
 }
 
