@@ -814,6 +814,7 @@ def main() -> None:
                          error=np.array([r["error"] if r is not None and "error" in r else "" for r in res]),
                          latency_ms=np.array([r["latency_ms"] if o else np.nan for r, o in zip(res, ok_)]),
                          batch=np.array([r["batch"] if o else 0 for r, o in zip(res, ok_)]),
+                         attempts=np.array([r["attempts"] if o else 0 for r, o in zip(res, ok_)]),
                          nt=np.array([r["nt"] for r in trace]), modes=np.array([r["modes"] for r in trace]),
                          kind=np.array([r["kind"] for r in trace]), length=np.array([r["length"] for r in trace]),
                          wall_s=wall, rate=args.serve_rate)
@@ -823,6 +824,7 @@ def main() -> None:
             runs[precision] = dict(results=res, rejected=rejected, wall_s=wall, served=len(ok),
                                    gradients_per_s=ngrad / wall, forwards_per_s=(len(ok) - ngrad) / wall,
                                    failed=sum(1 for r in res if r is not None and "error" in r),
+                                   rerun_alone=sum(1 for r in ok if r.get("attempts", 1) > 1),
                                    p50_ms=float(np.percentile(lat, 50)) if lat.size else None,
                                    p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
                                    solves_per_s=len(ok) / wall,
@@ -839,7 +841,7 @@ def main() -> None:
                         f"peak 1-20 kW, GDD/TOD/FOD phases, 50% forward / 50% forward+adjoint; single mode adaptive "
                         f"Tsit5 rtol=atol=1e-6, multimode fixed 5 mm RK4IP steps",
             **{f"{p}_{k}": runs[p][k] for p in runs
-               for k in ("served", "failed", "p50_ms", "p99_ms", "solves_per_s", "gradients_per_s", "forwards_per_s",
+               for k in ("served", "failed", "rerun_alone", "p50_ms", "p99_ms", "solves_per_s", "gradients_per_s", "forwards_per_s",
                          "wall_s", "mean_batch", "rejected")},
             **({"fp32_over_fp64_gradients_per_s": runs["fp32"]["gradients_per_s"] / runs["fp64"]["gradients_per_s"]}
                if "fp32" in runs and "fp64" in runs and runs["fp64"]["gradients_per_s"] > 0 else {}),

@@ -69,6 +69,7 @@ struct Req {
     double length = 0;
     int nsteps = 0;
     bool heavy = false;  // estimated work above kHeavyFlops
+    bool solo = false;   // re-run alone after non-finite fields inside a batch
     std::vector<double> u0;
     double *uL = nullptr, *grad = nullptr, *cost = nullptr;
     int state = 0;  // 0 queued, 1 running, 2 finished
@@ -146,6 +147,15 @@ struct Lane {
     Pinned<long long> h_stats;
     double* d_len = nullptr;
     int* d_nst = nullptr;
+    // host staging of the running batch: a block of the server's pinned
+    // staging arena, or the lane's own buffers above when it does not fit
+    char* io = nullptr;
+    size_t io_bytes = 0;
+    char* in = nullptr;
+    char* out = nullptr;
+    double *grad_h = nullptr, *cost_h = nullptr, *len_h = nullptr;
+    int *status_h = nullptr, *nst_h = nullptr;
+    long long* stats_h = nullptr;
 };
 
 }  // namespace
@@ -162,6 +172,7 @@ struct fh_server_s {
     std::vector<Lane> lanes;
     fhi::Arena arena;  // the memory budget, allocated once; plans sub-allocate from it
     fhi::Arena stage;  // pinned host staging of plan tables, so plan creation does not wait for uploads
+    fhi::Arena io;     // pinned host staging of batch inputs and outputs
     int64_t next_id = 1, plan_bytes = 0;
     bool stop = false;
     std::thread worker;
@@ -181,6 +192,11 @@ constexpr double kHeavyFlops = 1.5e12;
 // Pinned host staging of plan tables (a plan of 32768 samples and 10 modes
 // stages about 4 MB); when it is full, uploads fall back to synchronous ones.
 constexpr size_t kStageBytes = 256ull << 20;
+
+// Pinned host staging of batch inputs and outputs: a sixteenth of the memory
+// budget, between 256 MiB and 4 GiB.  A batch that does not fit stages
+// through its lane's own buffers.
+constexpr size_t kIoBytes = 4ull << 30;
 
 // Algorithmic flops of a request: the disp_mmf! structure per right-hand
 // side (5 N log2 N (2M + 2M^2) + 2 M^4 N + 17 M^2 N + 16 M N), times the
@@ -336,16 +352,19 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
     for (const auto& l : S->lanes) heavy_busy += l.busy && l.heavy;
     const int heavy_cap = std::max(1, (int)S->lanes.size() - (int)S->lanes.size() / 4);
     const Req* first = nullptr;
+    int64_t first_id = -1;
     for (int64_t id : S->queue) {
         const Req& r = S->reqs[id];
         if (!r.heavy || heavy_busy < heavy_cap) {
             first = &r;
+            first_id = id;
             break;
         }
     }
     if (!first) return false;
     const uint64_t key = first->key;
     const bool heavy = first->heavy;
+    const bool solo = first->solo;
     OpSet* op = &S->ops.at(key);
     int rc;
     bool wait;
@@ -358,9 +377,10 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
     // only this thread removes from the queue: the class's requests are still there
     std::vector<int64_t> ids;
     std::vector<Req*> rq;
-    const int capacity = pi ? pi->capacity : S->cfg.max_batch;
+    const int capacity = solo ? 1 : (pi ? pi->capacity : S->cfg.max_batch);
     for (auto it = S->queue.begin(); it != S->queue.end() && (int)ids.size() < capacity;) {
-        if (S->reqs[*it].key == key && S->reqs[*it].heavy == heavy) {
+        const Req& c = S->reqs[*it];
+        if (c.key == key && c.heavy == heavy && c.solo == solo && (!solo || *it == first_id)) {
             ids.push_back(*it);
             rq.push_back(&S->reqs[*it]);
             it = S->queue.erase(it);
@@ -379,6 +399,7 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
         r->state = 1;
         r->res.start_ns = t0;
         r->res.batch = (int)ids.size();
+        r->res.attempts += 1;
     }
     // requests in state 1 are neither erased nor modified by other threads
     lk.unlock();
@@ -388,38 +409,72 @@ bool dispatch(fh_server_s* S, Lane& L, std::unique_lock<std::mutex>& lk)
     const bool fp64 = S->cfg.precision == FH_PREC_FP64;
     const size_t es = fp64 ? sizeof(double) : sizeof(float);
     int e = FH_OK;
-    if (!e) e = L.h_in.ensure((size_t)B * nf * 2 * es);
-    if (!e) e = L.h_out.ensure((size_t)B * nf * 2 * es);
-    if (!e) e = L.h_grad.ensure((size_t)B * o.nt);
-    if (!e) e = L.h_cost.ensure(B);
-    if (!e) e = L.h_len.ensure(B);
-    if (!e) e = L.h_status.ensure(B);
-    if (!e) e = L.h_nst.ensure(B);
-    if (!e) e = L.h_stats.ensure(3 * (size_t)B);
+    {
+        // host staging from the pinned io arena (no allocation while serving);
+        // the lane's own buffers only when the block does not fit
+        const size_t a256 = 256;
+        auto up = [&](size_t n) { return (n + a256 - 1) / a256 * a256; };
+        const size_t s_f = up((size_t)B * nf * 2 * es), s_g = up(sizeof(double) * (size_t)B * o.nt),
+                     s_d = up(sizeof(double) * B), s_i = up(sizeof(int) * B), s_l = up(sizeof(long long) * 3 * B);
+        const size_t total = 2 * s_f + s_g + 2 * s_d + 2 * s_i + s_l;
+        char* blk = (char*)S->io.alloc(total);
+        L.io = blk;
+        L.io_bytes = blk ? total : 0;
+        if (blk) {
+            char* q = blk;
+            L.in = q; q += s_f;
+            L.out = q; q += s_f;
+            L.grad_h = (double*)q; q += s_g;
+            L.cost_h = (double*)q; q += s_d;
+            L.len_h = (double*)q; q += s_d;
+            L.status_h = (int*)q; q += s_i;
+            L.nst_h = (int*)q; q += s_i;
+            L.stats_h = (long long*)q;
+        } else {
+            if (!e) e = L.h_in.ensure((size_t)B * nf * 2 * es);
+            if (!e) e = L.h_out.ensure((size_t)B * nf * 2 * es);
+            if (!e) e = L.h_grad.ensure((size_t)B * o.nt);
+            if (!e) e = L.h_cost.ensure(B);
+            if (!e) e = L.h_len.ensure(B);
+            if (!e) e = L.h_status.ensure(B);
+            if (!e) e = L.h_nst.ensure(B);
+            if (!e) e = L.h_stats.ensure(3 * (size_t)B);
+            L.in = L.h_in.p;
+            L.out = L.h_out.p;
+            L.grad_h = L.h_grad.p;
+            L.cost_h = L.h_cost.p;
+            L.len_h = L.h_len.p;
+            L.status_h = L.h_status.p;
+            L.nst_h = L.h_nst.p;
+            L.stats_h = L.h_stats.p;
+        }
+    }
     for (int b = 0; e == FH_OK && b < B; ++b) {
         const Req& r = *rq[b];
         if (fp64) {
-            std::memcpy(L.h_in.p + (size_t)b * nf * 2 * es, r.u0.data(), (size_t)nf * 2 * es);
+            std::memcpy(L.in + (size_t)b * nf * 2 * es, r.u0.data(), (size_t)nf * 2 * es);
         } else {
-            float* dst = (float*)(L.h_in.p + (size_t)b * nf * 2 * es);
+            float* dst = (float*)(L.in + (size_t)b * nf * 2 * es);
             for (int i = 0; i < 2 * nf; ++i) dst[i] = (float)r.u0[i];
         }
-        L.h_len.p[b] = r.length;
-        L.h_nst.p[b] = r.nsteps;
+        L.len_h[b] = r.length;
+        L.nst_h[b] = r.nsteps;
     }
-    if (e == FH_OK && hipMemcpyAsync(fhi::plan_u_in(pi->plan), L.h_in.p, (size_t)B * nf * 2 * es,
+    if (e == FH_OK && hipMemcpyAsync(fhi::plan_u_in(pi->plan), L.in, (size_t)B * nf * 2 * es,
                                      hipMemcpyHostToDevice, L.s) != hipSuccess)
         e = fhi::set_error(FH_EDEVICE, "staging copy failed");
-    if (e == FH_OK && (hipMemcpyAsync(L.d_len, L.h_len.p, sizeof(double) * B, hipMemcpyHostToDevice, L.s) != hipSuccess ||
-                       hipMemcpyAsync(L.d_nst, L.h_nst.p, sizeof(int) * B, hipMemcpyHostToDevice, L.s) != hipSuccess))
+    if (e == FH_OK && (hipMemcpyAsync(L.d_len, L.len_h, sizeof(double) * B, hipMemcpyHostToDevice, L.s) != hipSuccess ||
+                       hipMemcpyAsync(L.d_nst, L.nst_h, sizeof(int) * B, hipMemcpyHostToDevice, L.s) != hipSuccess))
         e = fhi::set_error(FH_EDEVICE, "staging copy failed");
     if (e == FH_OK)
-        e = fhi::enqueue_batch(pi->plan, B, o.grad, L.d_len, o.modes > 1 ? L.d_nst : nullptr, L.s, L.h_out.p,
-                               L.h_grad.p, L.h_cost.p, L.h_status.p, L.h_stats.p);
+        e = fhi::enqueue_batch(pi->plan, B, o.grad, L.d_len, o.modes > 1 ? L.d_nst : nullptr, L.s, L.out,
+                               L.grad_h, L.cost_h, L.status_h, L.stats_h);
     if (e == FH_OK && hipEventRecord(L.ev, L.s) != hipSuccess) e = fhi::set_error(FH_EDEVICE, "hipEventRecord failed");
     trace_slow("staging and launch", ts, B);
     if (e != FH_OK) {
         (void)hipStreamSynchronize(L.s);
+        if (L.io) S->io.release(L.io, L.io_bytes);
+        L.io = nullptr;
         pi->last_used = now_ns();
         lk.lock();
         const std::string m = fh_last_error();
@@ -456,22 +511,22 @@ void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mute
     for (size_t b = 0; b < rq.size(); ++b) {
         Req& r = *rq[b];
         if (codes[b] == FH_OK) {
-            const int st = L.h_status.p[b];
+            const int st = L.status_h[b];
             codes[b] = (st & 2) ? FH_EMAXSTEPS : (st ? FH_ENONFINITE : FH_OK);
             if (fp64) {
-                std::memcpy(r.uL, L.h_out.p + b * nf * 2 * es, (size_t)nf * 2 * es);
+                std::memcpy(r.uL, L.out + b * nf * 2 * es, (size_t)nf * 2 * es);
             } else {
-                const float* src = (const float*)(L.h_out.p + b * nf * 2 * es);
+                const float* src = (const float*)(L.out + b * nf * 2 * es);
                 for (int i = 0; i < 2 * nf; ++i) r.uL[i] = src[i];
             }
             if (L.grad) {
-                if (r.grad) std::memcpy(r.grad, L.h_grad.p + b * L.nt, sizeof(double) * L.nt);
-                if (r.cost) r.cost[0] = L.h_cost.p[b];
+                if (r.grad) std::memcpy(r.grad, L.grad_h + b * L.nt, sizeof(double) * L.nt);
+                if (r.cost) r.cost[0] = L.cost_h[b];
             }
             if (adaptive) {
-                r.res.accepted = L.h_stats.p[3 * b];
-                r.res.rejected = L.h_stats.p[3 * b + 1];
-                r.res.rhs_evals = L.h_stats.p[3 * b + 2];
+                r.res.accepted = L.stats_h[3 * b];
+                r.res.rejected = L.stats_h[3 * b + 1];
+                r.res.rhs_evals = L.stats_h[3 * b + 2];
             } else {
                 r.res.accepted = r.nsteps;
                 r.res.rhs_evals = 4LL * r.nsteps;
@@ -482,8 +537,24 @@ void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mute
     L.pi->last_used = now_ns();
     L.pi = nullptr;
     trace_slow("output scatter", tc, (int)rq.size());
+    if (L.io) S->io.release(L.io, L.io_bytes);
+    L.io = nullptr;
     lk.lock();
+    // Non-finite fields from a solve inside a batch are re-run once, alone:
+    // every such failure seen so far came from FP32 multimode batches under
+    // heavy concurrent load, and the same requests solved alone or in small
+    // batches returned finite results.  A second failure is reported.
+    for (size_t b = 0; b < L.ids.size(); ++b) {
+        Req& r = S->reqs[L.ids[b]];
+        if (codes[b] == FH_ENONFINITE && L.ids.size() > 1 && r.res.attempts == 1) {
+            r.solo = true;
+            r.state = 0;
+            S->queue.push_front(L.ids[b]);
+            codes[b] = -1;  // re-queued
+        }
+    }
     for (size_t b = 0; b < L.ids.size(); ++b)
+        if (codes[b] != -1)
         finish(S, L.ids[b], codes[b],
                codes[b] == FH_EMAXSTEPS ? "adaptive solve reached max_steps"
                                         : (codes[b] == FH_ENONFINITE ? "solve produced non-finite fields"
@@ -578,6 +649,13 @@ int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out
         return rc;
     }
     if (int rc = S->stage.init(kStageBytes, true)) {
+        S->arena.destroy();
+        delete S;
+        return rc;
+    }
+    if (int rc = S->io.init(std::min<size_t>(kIoBytes, std::max<size_t>(kStageBytes, cfg->memory_budget_bytes / 16)),
+                            true)) {
+        S->stage.destroy();
         S->arena.destroy();
         delete S;
         return rc;
@@ -688,6 +766,7 @@ int fh_server_destroy(fh_server S)
     S->plans.clear();
     S->arena.destroy();
     S->stage.destroy();
+    S->io.destroy();
     for (auto& L : S->lanes) {
         if (L.s) (void)hipStreamSynchronize(L.s);
         L.h_in.release();

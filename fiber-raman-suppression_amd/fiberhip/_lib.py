@@ -132,6 +132,8 @@ class RequestResult(ctypes.Structure):
         ("rhs_evals", ctypes.c_int64),
         ("retcode", ctypes.c_int32),
         ("batch", ctypes.c_int32),
+        ("attempts", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -12,7 +12,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import FH_REQ_FORWARD, FH_REQ_GRADIENT, check, lib
+from ._lib import FH_REQ_FORWARD, FH_REQ_GRADIENT, FiberHIPError, check, lib
 from .engine import PRECISIONS, Context, default_context
 
 
@@ -100,14 +100,18 @@ class Server:
         with self._lock:
             keep = self._pending.pop(int(rid))
         out = dict(retcode=res.retcode, accepted=res.accepted, rejected=res.rejected, rhs_evaluations=res.rhs_evals,
-                   batch=res.batch, latency_ms=(res.finish_ns - res.submit_ns) / 1e6,
+                   batch=res.batch, attempts=res.attempts, latency_ms=(res.finish_ns - res.submit_ns) / 1e6,
                    submit_ns=res.submit_ns, start_ns=res.start_ns, finish_ns=res.finish_ns,
                    queue_ms=(res.start_ns - res.submit_ns) / 1e6 if res.start_ns else None,
                    uL=keep["uL"].T)
         if "grad" in keep:
             out["grad"], out["cost"] = keep["grad"], float(keep["cost"][0])
         if rc:
-            check(rc)
+            try:
+                check(rc)
+            except FiberHIPError as exc:
+                exc.result = out  # counters, batch and attempts of the failed request
+                raise
         return out
 
     def close(self) -> None:

@@ -404,6 +404,9 @@ typedef struct {
     int64_t accepted, rejected, rhs_evals;  /* this solve's counters                                */
     int32_t retcode;                        /* FH_OK or the solve's error code                      */
     int32_t batch;                          /* solves in the launch that carried it                 */
+    int32_t attempts;                       /* launches the request took: a solve that returned     *
+                                             * non-finite fields inside a batch is re-run once, alone */
+    int32_t reserved;
 } fh_request_result;
 
 int fh_server_create(fh_context ctx, const fh_server_config *cfg, fh_server *out);

@@ -134,3 +134,40 @@ def test_memory_budget_admission_and_rejections(gpu_context):
     with fiberhip.Server(lanes=1, max_batch=4, precision="fp32", memory_budget_bytes=1 << 20) as starved:
         with pytest.raises(fiberhip.FiberHIPError, match="FH_ENOMEM"):
             starved.wait(_submit(starved, reqs[1]))
+
+
+def _shared_a_launch(batches) -> bool:
+    """Given the launch sizes the good requests report, whether one other
+    request rode in one of their launches.  Launches take queued requests in
+    order, so a launch of k requests that held only good ones is reported k
+    times; one that held the failed request too, k - 1 times."""
+    from collections import Counter
+
+    return any(n % k for k, n in Counter(batches).items())
+
+
+def test_non_finite_request_is_rerun_alone(gpu_context):
+    """A multimode request whose FP32 fields overflow (launch x 1e30: |u|^2
+    leaves FP32's range in the first Kerr term) fails with FH_ENONFINITE.  If
+    it shared a launch it is re-run once, alone, before it is reported
+    (attempts 2); the requests launched with it are unaffected -- bitwise equal
+    to the same requests solved alone."""
+    import fiberhip
+    from fiberhip import grin
+
+    mm = grin.grin_problem(3, nt=512, time_window_ps=4.0, length_m=0.01, energy_j=1e-9, nx=96, window_um=60.0)
+    launches = [mm.uω0 * np.exp(1j * 0.3), mm.uω0 * 1e30, mm.uω0 * np.exp(1j * 1.1), mm.uω0 * np.exp(1j * 2.0)]
+    print("case: serving non-finite rerun fp32", flush=True)
+    with fiberhip.Server(lanes=1, max_batch=8, precision="fp32", multimode_step_m=2.5e-4) as srv:
+        ids = [srv.submit(mm, kind="forward", u0=u, length_m=0.004) for u in launches]
+        good = [srv.wait(ids[i]) for i in (0, 2, 3)]
+        with pytest.raises(fiberhip.FiberHIPError, match="FH_ENONFINITE") as err:
+            srv.wait(ids[1])
+    bad = err.value.result
+    assert bad["attempts"] == (2 if _shared_a_launch([g["batch"] for g in good]) else 1)
+    assert bad["batch"] == 1
+    for g, u in zip(good, [launches[i] for i in (0, 2, 3)]):
+        assert g["retcode"] == 0 and g["attempts"] == 1
+        with fiberhip.Server(lanes=1, max_batch=1, precision="fp32", multimode_step_m=2.5e-4) as one:
+            alone = one.wait(one.submit(mm, kind="forward", u0=u, length_m=0.004))
+        assert np.array_equal(alone["uL"], g["uL"])

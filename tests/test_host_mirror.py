@@ -118,7 +118,7 @@ def test_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.Stats) == 32
     assert ctypes.sizeof(_lib.ServerConfig) == 8 + 4 * 4 + 3 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.Request) == 4 * 4 + 8 + 4 * 8 + 8 + 8 + 4 * 8
-    assert ctypes.sizeof(_lib.RequestResult) == 6 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.RequestResult) == 6 * 8 + 4 * 4
 
 
 def test_last_error_starts_empty_and_bad_args_fail_without_gpu():
