@@ -175,6 +175,9 @@ _SIGNATURES = {
     "fh_forward_gain": ([_VP, ctypes.c_int32, ctypes.POINTER(GainDesc), _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)],
                         ctypes.c_int),
     "fh_field_metrics": ([_VP, ctypes.c_int32, _DP, _DP, ctypes.c_double, _DP], ctypes.c_int),
+    "fh_fd_modes": ([_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _DP, ctypes.c_int32,
+                     ctypes.c_double, ctypes.c_int32, _DP, _DP, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "fh_overlap_tensor": ([_VP, ctypes.c_int64, ctypes.c_int32, _DP, ctypes.c_double, _DP], ctypes.c_int),
     "fh_forward_device": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "fh_energy_device": ([_VP, ctypes.c_int32, _VP, _DP], ctypes.c_int),
     "fh_gradient": ([_VP, ctypes.c_int32, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(Stats)], ctypes.c_int),

@@ -75,11 +75,16 @@ def hg_modes(modes: int, wavelength_um=1.55, radius_um=25.0, core_na=0.2, nx=256
     return out, x, dx
 
 
-def overlap_tensor(modes: np.ndarray, dx_si: float) -> np.ndarray:
-    """S_K[i,j,k,l] = sum_mn phi_i phi_j phi_k phi_l / dx^2 (fibers.jl:214-222)."""
+def overlap_tensor(modes: np.ndarray, dx_si: float, backend: str = "host") -> np.ndarray:
+    """S_K[i,j,k,l] = sum_mn phi_i phi_j phi_k phi_l / dx^2 (fibers.jl:214-222).
+    backend "device": FP64 on the GPU (fh_overlap_tensor)."""
     nx = modes.shape[0]
     f = modes.reshape(nx * nx, -1)
     m = f.shape[1]
+    if backend == "device":
+        return _overlap_tensor_device(f, 1.0 / dx_si ** 2)
+    if backend != "host":
+        raise ValueError('backend must be "host" or "device"')
     pair = np.einsum("ri,rj->rij", f, f).reshape(nx * nx, m * m)
     return (pair.T @ pair).reshape(m, m, m, m) / dx_si ** 2
 
@@ -110,20 +115,23 @@ def grin_dispersion(sim: dict, modes: int, beta_order=2, delta_f_thz=1.0, radius
 def grin_problem(modes: int = 6, nt: int = 4096, time_window_ps: float = 12.0, length_m: float = 0.3,
                  energy_j: float = 10e-9, fwhm_s: float = 100e-15, seed: int = 1, nx: int = 256,
                  window_um: float = 80.0, radius_um=25.0, core_na=0.2, wavelength_m=1550e-9, fR=0.18,
-                 launch=None, mode_solver: str = "hg") -> FiberFieldProblem:
+                 launch=None, mode_solver: str = "hg", setup_backend: str = "host") -> FiberFieldProblem:
     """Default problem: parabolic GRIN (50 um core, NA 0.2), M modes on an
     nx^2 grid, sech^2 pulse of energy_j split over modes by Dirichlet(1)
     weights with uniform phases (seeded), initial_modes normalised
     (physics_models.jl:289-297).  mode_solver "hg" uses the analytic
     Hermite-Gaussian modes and beta_pq; "fd" the reference's procedure
-    (get_params: finite-difference eigensolve of the index profile)."""
+    (get_params: finite-difference eigensolve of the index profile).
+    setup_backend "device" runs the mode solves and the overlap tensor on the
+    GPU (fh_fd_modes, fh_overlap_tensor)."""
     sim = get_disp_sim_params(wavelength_m, modes, nt, time_window_ps, 2)
     w0 = 2 * math.pi * sim["f0"] * 1e12
     if mode_solver == "fd":
-        _, D, gamma, phi, _ = get_params(sim["f0"], nx, window_um, radius_um, core_na, 2.0, modes, nt, sim["Δt"], 2)
+        _, D, gamma, phi, _ = get_params(sim["f0"], nx, window_um, radius_um, core_na, 2.0, modes, nt, sim["Δt"], 2,
+                                         backend=setup_backend)
     elif mode_solver == "hg":
         phi, _, dx = hg_modes(modes, wavelength_m * 1e6, radius_um, core_na, nx, window_um)
-        sk = overlap_tensor(phi, dx * 1e-6)
+        sk = overlap_tensor(phi, dx * 1e-6, setup_backend)
         gamma = sk * N2_SILICA * w0 / 2.99792458e8
         D = grin_dispersion(sim, modes, radius_um=radius_um, core_na=core_na)
     else:
@@ -169,11 +177,18 @@ def build_grin(wavelength_um: float, nx: int, window_um: float, radius_um: float
     return eps, x, dx
 
 
-def solve_for_fiber_modes(wavelength_um: float, nmodes: int, dx: float, eps: np.ndarray):
+def solve_for_fiber_modes(wavelength_um: float, nmodes: int, dx: float, eps: np.ndarray, backend: str = "host"):
     """solve_for_fiber_modes, fibers.jl:159-197: the nmodes largest eigenpairs of
     A = laplacian_5pt + eps (2 pi / lambda)^2 (zero field outside the grid).
     Returns beta^2 [um^-2] (descending), unit-norm modes (nx*nx, nmodes) in
-    column-major grid order, and n_eff = lambda sqrt(beta^2) / (2 pi)."""
+    column-major grid order, and n_eff = lambda sqrt(beta^2) / (2 pi).
+    backend "host": scipy's shift-invert Lanczos; "device": Chebyshev-filtered
+    subspace iteration in FP64 on the GPU (fh_fd_modes)."""
+    if backend == "device":
+        d, v = _fd_modes_device(eps, 2 * math.pi / wavelength_um, dx, nmodes)
+        return d, v, wavelength_um * np.sqrt(d) / (2 * math.pi)
+    if backend != "host":
+        raise ValueError('backend must be "host" or "device"')
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
 
@@ -194,6 +209,40 @@ def solve_for_fiber_modes(wavelength_um: float, nmodes: int, dx: float, eps: np.
     return d, v, wavelength_um * np.sqrt(d) / (2 * math.pi)
 
 
+def _fd_modes_device(eps: np.ndarray, k0: float, dx: float, nmodes: int, tol: float = 1e-10, max_iter: int = 200):
+    """fh_fd_modes: (beta^2 descending, modes (nx*ny, nmodes) column-major grid order)."""
+    import ctypes
+
+    from ._lib import check, lib
+    from .engine import default_context
+
+    nx, ny = eps.shape
+    e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(-1, order="F"))
+    d = np.empty(nmodes)
+    v = np.empty((nmodes, nx * ny))
+    its = ctypes.c_int32()
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().fh_fd_modes(default_context().handle, nx, ny, float(dx), float(k0), e.ctypes.data_as(dp), int(nmodes),
+                            float(tol), int(max_iter), d.ctypes.data_as(dp), v.ctypes.data_as(dp), ctypes.byref(its)))
+    return d, v.T
+
+
+def _overlap_tensor_device(f: np.ndarray, scale: float) -> np.ndarray:
+    """fh_overlap_tensor of the modes f (points, M)."""
+    import ctypes
+
+    from ._lib import check, lib
+    from .engine import default_context
+
+    points, m = f.shape
+    phi = np.ascontiguousarray(np.asarray(f, dtype=np.float64).T)
+    sk = np.empty(m ** 4)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().fh_overlap_tensor(default_context().handle, int(points), int(m), phi.ctypes.data_as(dp),
+                                  float(scale), sk.ctypes.data_as(dp)))
+    return sk.reshape(m, m, m, m, order="F")
+
+
 def central_fd_weights(points: int, order: int) -> np.ndarray:
     """Weights w on the offsets -(points-1)/2 .. (points-1)/2 with
     sum_j w_j o_j^m / m! = [m == order] for m < points (central_fdm(points, order).coefs)."""
@@ -206,9 +255,10 @@ def central_fd_weights(points: int, order: int) -> np.ndarray:
 
 
 def get_params(f0_thz: float, nx: int, window_um: float, radius_um: float, core_na: float, alpha: float, modes: int,
-               nt: int, dt_ps: float, beta_order: int, delta_f_thz: float = 1.0):
+               nt: int, dt_ps: float, beta_order: int, delta_f_thz: float = 1.0, backend: str = "host"):
     """get_params, fibers.jl:91-132: (betas (beta_order+1, M), D (nt, M) rad/m,
-    gamma (M,M,M,M) 1/(W m), modes (nx, nx, M), x) from finite-difference modes."""
+    gamma (M,M,M,M) 1/(W m), modes (nx, nx, M), x) from finite-difference modes.
+    backend "device" runs the mode solves and the overlap tensor on the GPU."""
     c0 = 2.99792458e8
     points = 2 * beta_order + 1
     half = (points - 1) // 2
@@ -216,7 +266,7 @@ def get_params(f0_thz: float, nx: int, window_um: float, radius_um: float, core_
     for i, off in enumerate(range(-half, half + 1)):
         lam_um = c0 / ((f0_thz + delta_f_thz * off) * 1e12) * 1e6
         eps, _, dx = build_grin(lam_um, nx, window_um, radius_um, core_na, alpha)
-        _, _, neff = solve_for_fiber_modes(lam_um, modes, dx, eps)
+        _, _, neff = solve_for_fiber_modes(lam_um, modes, dx, eps, backend)
         beta_f[i] = 2 * math.pi * neff / (lam_um * 1e-6)
     dbeta = np.stack([central_fd_weights(points, n) @ beta_f / (2 * math.pi * delta_f_thz / 1e-12) ** n
                       for n in range(1, beta_order + 1)])
@@ -225,8 +275,8 @@ def get_params(f0_thz: float, nx: int, window_um: float, radius_um: float, core_
     D = np.stack([w ** n / math.factorial(n) for n in range(beta_order + 1)], axis=1) @ betas
     lam0_um = c0 / f0_thz / 1e12 * 1e6
     eps, x, dx = build_grin(lam0_um, nx, window_um, radius_um, core_na, alpha)
-    _, phi, _ = solve_for_fiber_modes(lam0_um, modes, dx, eps)
+    _, phi, _ = solve_for_fiber_modes(lam0_um, modes, dx, eps, backend)
     phi = phi.reshape(nx, nx, modes, order="F")
-    sk = overlap_tensor(phi, dx * 1e-6)
+    sk = overlap_tensor(phi, dx * 1e-6, backend)
     gamma = sk * N2_SILICA * 2 * math.pi * f0_thz * 1e12 / c0
     return betas, D, gamma, phi, x

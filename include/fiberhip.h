@@ -308,6 +308,29 @@ int fh_energy_device(fh_plan plan, int32_t batch, const void *d_u, double *energ
 int fh_field_metrics(fh_plan plan, int32_t count, const double *spectra, const double *omega, double dt_ps,
                      double *out);
 
+/* ------------------------------------------------------- GRIN modal setup */
+
+/* Guided modes of the scalar wave equation on an nx x ny grid, FP64 on the
+ * device (replaces solve_for_fiber_modes, src/simulation/fibers.jl:159-197):
+ * the nmodes largest eigenpairs of A = k0^2 diag(eps) + (5-point Laplacian)/dx^2
+ * with zero field outside the grid (eigs(A; nev = nmodes, which = :LR)), by
+ * Chebyshev-filtered subspace iteration.  eps: [ny][nx] host, first grid index
+ * fastest (Julia's column-major eps); dx and 1/k0 in the same length unit.
+ * Outputs: beta2[nmodes] descending (the eigenvalues beta^2), modes
+ * [nmodes][nx*ny] (unit 2-norm, same grid order, largest entry positive; within
+ * a degenerate group the basis is arbitrary, as with eigs), iterations (may be
+ * NULL).  tol: relative residual ||A v - beta^2 v|| / beta^2 (<= 0: 1e-10);
+ * max_iter <= 0: 200.  nmodes <= 40.  FH_EMAXSTEPS, with the outputs written,
+ * when the tolerance is not met. */
+int fh_fd_modes(fh_context ctx, int32_t nx, int32_t ny, double dx, double k0, const double *eps, int32_t nmodes,
+                double tol, int32_t max_iter, double *beta2, double *modes, int32_t *iterations);
+
+/* Overlap tensor of `modes` real sampled modes, FP64 on the device (replaces
+ * compute_overlap_tensor, src/simulation/fibers.jl:214-222):
+ * sk[i + M (j + M (k + M l))] = scale * sum_r phi_i(r) phi_j(r) phi_k(r) phi_l(r)
+ * for phi [modes][points] (host); scale = 1 / dx^2 there.  modes <= 16. */
+int fh_overlap_tensor(fh_context ctx, int64_t points, int32_t modes, const double *phi, double scale, double *sk);
+
 /* Device buffers on the context's device (plain hipMalloc / hipMemcpy on the
  * context stream).  Callers without their own HIP allocator use these to
  * keep fields resident between calls. */
