@@ -33,6 +33,8 @@ constexpr int kMaxVecs = 48;     // subspace size bound: nmodes + kGuard
 constexpr int kGuard = 8;        // vectors beyond the wanted modes
 constexpr int kDegree = 40;      // Chebyshev filter degree
 constexpr int kGramBlocks = 256; // workgroups of the overlap reduction
+constexpr int kParts = 256;      // workgroups (partial sums) of the subspace Gram and residual kernels
+constexpr int kMaxOwnGram = (kMaxVecs * kMaxVecs + kBlock - 1) / kBlock;
 constexpr int kMaxOverlapModes = 16;
 constexpr int kMaxOwn = (kMaxOverlapModes * (kMaxOverlapModes + 1) / 2 * kMaxOverlapModes * (kMaxOverlapModes + 1) / 2 +
                          kBlock - 1) / kBlock;
@@ -84,40 +86,67 @@ __global__ __launch_bounds__(kBlock) void k_cheb(const double* __restrict__ x, c
     y[off + r] = v;
 }
 
-// partial[t][i q + j] = sum over rows of tile t of X_i(r) Y_j(r) (i < p, j < q).
+// partial[b][i q + j] = sum of X_i(r) Y_j(r) (i < p, j < q) over the row
+// tiles b, b + G, ... of workgroup b (G = gridDim.x), accumulated in tile
+// order; thread t owns the entries t + kBlock e.
 __global__ __launch_bounds__(kBlock) void k_gram_partial(const double* __restrict__ X, int p,
                                                          const double* __restrict__ Y, int q, int64_t n,
                                                          double* __restrict__ partial)
 {
     __shared__ double xs[kMaxVecs * kRows], ys[kMaxVecs * kRows];
-    const int64_t r0 = (int64_t)blockIdx.x * kRows;
-    for (int t = threadIdx.x; t < p * kRows; t += kBlock) {
-        const int v = t / kRows, rr = t % kRows;
-        xs[t] = r0 + rr < n ? X[(int64_t)v * n + r0 + rr] : 0.0;
+    const int pq = p * q;
+    double acc[kMaxOwnGram];
+#pragma unroll
+    for (int e = 0; e < kMaxOwnGram; ++e) acc[e] = 0.0;
+    const int64_t tiles = (n + kRows - 1) / kRows;
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t r0 = tile * kRows;
+        __syncthreads();
+        for (int t = threadIdx.x; t < p * kRows; t += kBlock) {
+            const int v = t / kRows, rr = t % kRows;
+            xs[t] = r0 + rr < n ? X[(int64_t)v * n + r0 + rr] : 0.0;
+        }
+        for (int t = threadIdx.x; t < q * kRows; t += kBlock) {
+            const int v = t / kRows, rr = t % kRows;
+            ys[t] = r0 + rr < n ? Y[(int64_t)v * n + r0 + rr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kMaxOwnGram; ++e) {
+            const int t = threadIdx.x + kBlock * e;
+            if (t < pq) {
+                const int i = t / q, j = t % q;
+                double s = 0.0;
+                for (int rr = 0; rr < kRows; ++rr) s += xs[i * kRows + rr] * ys[j * kRows + rr];
+                acc[e] += s;
+            }
+        }
     }
-    for (int t = threadIdx.x; t < q * kRows; t += kBlock) {
-        const int v = t / kRows, rr = t % kRows;
-        ys[t] = r0 + rr < n ? Y[(int64_t)v * n + r0 + rr] : 0.0;
-    }
-    __syncthreads();
-    double* out = partial + (int64_t)blockIdx.x * p * q;
-    for (int t = threadIdx.x; t < p * q; t += kBlock) {
-        const int i = t / q, j = t % q;
-        double s = 0.0;
-        for (int rr = 0; rr < kRows; ++rr) s += xs[i * kRows + rr] * ys[j * kRows + rr];
-        out[t] = s;
+#pragma unroll
+    for (int e = 0; e < kMaxOwnGram; ++e) {
+        const int t = threadIdx.x + kBlock * e;
+        if (t < pq) partial[(int64_t)blockIdx.x * pq + t] = acc[e];
     }
 }
 
-// out[k] = sum over t < parts of partial[t][k], in order of t.
+// out[k] = sum over t < parts of partial[t][k] for k = blockIdx.x: lane l
+// adds the partials l, l + kBlock, ... in order, then a fixed tree adds the
+// lanes.
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial, int64_t parts, int m,
                                                          double* __restrict__ out)
 {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= m) return;
+    __shared__ double red[kBlock];
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int64_t t = 0; t < parts; ++t) s += partial[t * m + k];
-    out[k] = s;
+    for (int64_t t = threadIdx.x; t < parts; t += kBlock) s += partial[t * m + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+#pragma unroll 1
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
 }
 
 // X_j(r) = sum_i Y_i(r) C[i q + j] for j < q (X and Y distinct).
@@ -136,21 +165,34 @@ __global__ __launch_bounds__(kBlock) void k_combine(const double* __restrict__ Y
     }
 }
 
-// partial[t][j] = sum over rows of tile t of (AX_j - w_j X_j)^2, j < m.
+// partial[b][j] = sum of (AX_j - w_j X_j)^2 (j < m) over the rows of the row
+// tiles b, b + G, ... of workgroup b, in order; lane r of the workgroup takes
+// row r of each tile and the lanes' sums are added in lane order.
 __global__ __launch_bounds__(kBlock) void k_resid_partial(const double* __restrict__ AX, const double* __restrict__ X,
                                                           const double* __restrict__ w, int m, int64_t n,
                                                           double* __restrict__ partial)
 {
-    const int j = threadIdx.x;
-    if (j >= m) return;
-    const int64_t r0 = (int64_t)blockIdx.x * kRows;
-    double s = 0.0;
-    for (int rr = 0; rr < kRows && r0 + rr < n; ++rr) {
-        const int64_t k = (int64_t)j * n + r0 + rr;
-        const double e = AX[k] - w[j] * X[k];
-        s += e * e;
+    __shared__ double red[kBlock];
+    const int64_t tiles = (n + kBlock - 1) / kBlock;
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+            const int64_t r = tile * kBlock + threadIdx.x;
+            if (r < n) {
+                const int64_t k = (int64_t)j * n + r;
+                const double e = AX[k] - w[j] * X[k];
+                s += e * e;
+            }
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int l = 0; l < kBlock; ++l) t += red[l];
+            partial[(int64_t)blockIdx.x * m + j] = t;
+        }
+        __syncthreads();
     }
-    partial[(int64_t)blockIdx.x * m + j] = s;
 }
 
 // Deterministic start block: splitmix64 of the element index, uniform in [-1, 1).
@@ -288,9 +330,9 @@ struct Block {
     int nx, ny;
     const double* d0;
     double c;
-    double* partial;  // [tiles][p^2]
+    double* partial;  // [parts][p^2]
     double* small;    // [kMaxVecs^2] reduced Gram / coefficients
-    int64_t tiles() const { return (n + kRows - 1) / kRows; }
+    int64_t parts() const { return std::min<int64_t>(kParts, (n + kRows - 1) / kRows); }
     dim3 rows_grid() const { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
     int apply(const double* x, const double* z, double* y, int p, double cen, double a, double b)
@@ -303,10 +345,10 @@ struct Block {
     // g = X^T Y (p x q, row-major) on the host
     int gram(const double* X, int p, const double* Y, int q, std::vector<double>& g)
     {
-        hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)tiles()), dim3(kBlock), 0, s, X, p, Y, q, n, partial);
+        hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)parts()), dim3(kBlock), 0, s, X, p, Y, q, n, partial);
         GR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((p * q + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                           partial, tiles(), p * q, small);
+        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)(p * q)), dim3(kBlock), 0, s, partial, parts(), p * q,
+                           small);
         GR_HIP(hipGetLastError());
         g.resize((size_t)p * q);
         GR_HIP(hipMemcpyAsync(g.data(), small, sizeof(double) * p * q, hipMemcpyDeviceToHost, s));
@@ -391,8 +433,7 @@ int fh_fd_modes(fh_context ctx, int32_t nx, int32_t ny, double dx, double k0, co
     if ((rc = dd0.alloc(n))) return rc;
     for (auto& b : buf)
         if ((rc = b.alloc((size_t)p * n))) return rc;
-    const int64_t tiles = (n + kRows - 1) / kRows;
-    if ((rc = part.alloc((size_t)tiles * p * p))) return rc;
+    if ((rc = part.alloc((size_t)kParts * p * p))) return rc;
     if ((rc = small.alloc(kMaxVecs * kMaxVecs))) return rc;
     Block B{ctx->stream, n, nx, ny, dd0.p, c, part.p, small.p};
     GR_HIP(hipMemcpyAsync(dd0.p, d0.data(), sizeof(double) * n, hipMemcpyHostToDevice, B.s));
@@ -452,10 +493,10 @@ int fh_fd_modes(fh_context ctx, int32_t nx, int32_t ny, double dx, double k0, co
             if (b != X) (o++ == 0 ? T1 : T2) = b;
         // residuals of the wanted pairs
         GR_HIP(hipMemcpyAsync(small.p, w.data(), sizeof(double) * nmodes, hipMemcpyHostToDevice, B.s));
-        hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)tiles), dim3(kBlock), 0, B.s, AX, X, small.p, nmodes, n,
-                           part.p);
+        hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)B.parts()), dim3(kBlock), 0, B.s, AX, X, small.p, nmodes,
+                           n, part.p);
         GR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, B.s, part.p, tiles, nmodes,
+        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nmodes), dim3(kBlock), 0, B.s, part.p, B.parts(), nmodes,
                            small.p + kMaxVecs);
         GR_HIP(hipGetLastError());
         GR_HIP(hipMemcpyAsync(res.data(), small.p + kMaxVecs, sizeof(double) * nmodes, hipMemcpyDeviceToHost, B.s));
@@ -524,8 +565,8 @@ int fh_overlap_tensor(fh_context ctx, int64_t points, int32_t modes, const doubl
         hipLaunchKernelGGL(k_overlap_partial, dim3(kGramBlocks), dim3(kBlock), 0, s, dphi.p, M, (int64_t)points,
                            dpairs, Q, dpart.p);
         if ((x = hipGetLastError()) != hipSuccess) return x;
-        hipLaunchKernelGGL(k_sum_partials, dim3((QQ + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dpart.p,
-                           (int64_t)kGramBlocks, QQ, dsum.p);
+        hipLaunchKernelGGL(k_sum_partials, dim3(QQ), dim3(kBlock), 0, s, dpart.p, (int64_t)kGramBlocks, QQ,
+                           dsum.p);
         if ((x = hipGetLastError()) != hipSuccess) return x;
         if ((x = hipMemcpyAsync(g.data(), dsum.p, sizeof(double) * QQ, hipMemcpyDeviceToHost, s)) != hipSuccess)
             return x;
