@@ -116,8 +116,9 @@ template <typename T> struct Pinned {
     }
 };
 
-// Opt-in trace (environment FH_SERVER_TRACE set): scheduler steps that took
-// longer than 20 ms, to stderr.
+// Opt-in trace (environment FH_SERVER_TRACE set), to stderr: scheduler steps
+// that took longer than 20 ms, and every launch that returned a non-finite
+// solve, member by member.
 bool trace_on()
 {
     static const bool on = std::getenv("FH_SERVER_TRACE") != nullptr;
@@ -533,6 +534,28 @@ void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mute
             }
         }
     }
+    if (trace_on()) {
+        // one line per launch that returned a non-finite solve: every member's
+        // status, length and count of non-finite output entries
+        bool any = false;
+        for (int c : codes) any |= c == FH_ENONFINITE;
+        if (any) {
+            std::string line = "fh_server: launch of " + std::to_string(rq.size()) + " (nt " + std::to_string(L.nt) +
+                               ", modes " + std::to_string(L.modes) + (L.grad ? ", gradient" : ", forward") + "):";
+            for (size_t b = 0; b < rq.size(); ++b) {
+                const Req& r = *rq[b];
+                int bad_u = 0, bad_g = 0;
+                for (int i = 0; i < 2 * nf; ++i) bad_u += !std::isfinite(r.uL[i]);
+                if (L.grad && r.grad)
+                    for (int i = 0; i < L.nt; ++i) bad_g += !std::isfinite(r.grad[i]);
+                char buf[160];
+                std::snprintf(buf, sizeof buf, " [id %lld L %.3f status %d uL-nonfinite %d grad-nonfinite %d attempt %d]",
+                              (long long)L.ids[b], r.length, L.status_h[b], bad_u, bad_g, r.res.attempts);
+                line += buf;
+            }
+            std::fprintf(stderr, "%s\n", line.c_str());
+        }
+    }
     L.pi->busy = false;
     L.pi->last_used = now_ns();
     L.pi = nullptr;
@@ -647,6 +670,17 @@ int fh_server_create(fh_context ctx, const fh_server_config* cfg, fh_server* out
     if (int rc = S->arena.init((size_t)cfg->memory_budget_bytes)) {
         delete S;
         return rc;
+    }
+    // Diagnostic (environment FH_SERVER_FILL=<byte>): fill the device arena
+    // with that byte, so a kernel that reads plan memory it has not written
+    // sees it (0xff reads as NaN in FP32 and FP64).
+    if (const char* f = std::getenv("FH_SERVER_FILL")) {
+        const int byte = std::atoi(f) & 0xff;
+        if (hipMemset(S->arena.base, byte, S->arena.size) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            S->arena.destroy();
+            delete S;
+            return fhi::set_error(FH_EDEVICE, "filling the server's device memory failed");
+        }
     }
     if (int rc = S->stage.init(kStageBytes, true)) {
         S->arena.destroy();
