@@ -33,7 +33,6 @@ sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 # launches.  HIP reads the value once, when it initialises, so main() sets it
 # before any HIP call, overriding an inherited value.
 SERVE_LANES = 16
-HW_QUEUES = SERVE_LANES + 4
 
 import numpy as np  # noqa: E402
 
@@ -548,6 +547,8 @@ def main() -> None:
     ap.add_argument("--serve-requests", type=int, default=100, help="requests in the serving trace")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-budget-gib", type=float, default=160.0, help="device memory of the server's plans")
+    ap.add_argument("--serve-lanes", type=int, default=SERVE_LANES,
+                    help="server lanes (concurrent launches); the process gets lanes + 4 hardware queues")
     ap.add_argument("--serve-precisions", default="fp32,fp64", help="server precisions to run, comma-separated")
     ap.add_argument("--serve-first", type=int, default=0,
                     help="serve only the first K requests of the --serve-requests trace (same requests and arrivals)")
@@ -559,7 +560,7 @@ def main() -> None:
     ap.add_argument("--serve-report", default="",
                     help="summarise the records PREFIX_fp32.npz and PREFIX_fp64.npz written by --serve-save (no GPU)")
     args = ap.parse_args()
-    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.serve_lanes + 4)
     if args.serve_report:
         print(json.dumps(serving_report(args.serve_report)), flush=True)
         return
@@ -801,7 +802,8 @@ def main() -> None:
         cache = {}
         runs = {}
         for precision in args.serve_precisions.split(","):
-            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache, budget_gib=args.serve_budget_gib,
+            res, rejected, wall = run_serving(trace, precision, args.serve_rate, cache, lanes=args.serve_lanes,
+                                              budget_gib=args.serve_budget_gib,
                                               drop_fields=bool(args.serve_save))
             if args.serve_save:
                 ok_ = [r is not None and "error" not in r for r in res]
@@ -845,7 +847,7 @@ def main() -> None:
                          "wall_s", "mean_batch", "rejected")},
             **({"fp32_over_fp64_gradients_per_s": runs["fp32"]["gradients_per_s"] / runs["fp64"]["gradients_per_s"]}
                if "fp32" in runs and "fp64" in runs and runs["fp64"]["gradients_per_s"] > 0 else {}),
-            "lanes": SERVE_LANES, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+            "lanes": args.serve_lanes, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
             "memory_budget_gib": args.serve_budget_gib,
             "latency": "from each request's scheduled Poisson arrival to its completion",
             "fp32_vs_fp64_matched_requests": len(errs),
