@@ -24,7 +24,10 @@ class Server:
     """A libfiberhip request server on one context.
 
     memory_budget_bytes bounds the device memory of the server's plans; lanes
-    is the number of launches in flight; max_batch the solves per launch.
+    is the number of launches in flight (the process needs lanes + 4 hardware
+    queues, GPU_MAX_HW_QUEUES; on MI355X keep lanes at 12 or below: with 16 and
+    more, long FP32 solves came back non-finite under load); max_batch the
+    solves per launch.
     Single-mode requests use the adaptive Tsit5 scheme (rtol, atol), multimode
     requests the fixed-step RK4IP with ceil(L / multimode_step_m) steps (at
     most mm_grad_max_steps for multimode gradients).  Single-mode gradient plans
