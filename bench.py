@@ -31,10 +31,10 @@ sys.path[:0] = [str(ROOT / "fiber-raman-suppression_amd"), str(ROOT)]
 # Many more active queues oversubscribe the hardware scheduler: with 28
 # lanes on 32 queues a plan's table upload waited up to 160 s behind running
 # launches.  HIP reads the value once, when it initialises, so main() sets it
-# before any HIP call, overriding an inherited value.  12 lanes served the
-# 10,000-request trace with no non-finite solve.  16 and 20 lanes gave the
-# same throughput, but long FP32 solves of the 1024-thread kernels came back
-# non-finite in 2-3 and about 15 launches.
+# before any HIP call, overriding an inherited value.  On the 10,000-request
+# trace, 12, 16 and 20 lanes give the same throughput, but launches of the
+# 1024-thread FP32 kernels came back non-finite in 0-5, 2-3 and about 15
+# launches per run.
 SERVE_LANES = 12
 
 import numpy as np  # noqa: E402

@@ -556,8 +556,23 @@ void complete(fh_server_s* S, Lane& L, int device_rc, std::unique_lock<std::mute
             std::fprintf(stderr, "%s\n", line.c_str());
         }
     }
-    L.pi->busy = false;
-    L.pi->last_used = now_ns();
+    // A plan instance whose launch returned a non-finite solve is released
+    // instead of reused: its re-runs and later requests of the class get a
+    // plan with freshly uploaded tables.
+    bool quarantine = false;
+    for (int c : codes) quarantine |= c == FH_ENONFINITE;
+    if (quarantine) {
+        for (auto it = S->plans.begin(); it != S->plans.end(); ++it)
+            if (&*it == L.pi) {
+                S->plan_bytes -= it->bytes;
+                fh_plan_destroy(it->plan);
+                S->plans.erase(it);
+                break;
+            }
+    } else {
+        L.pi->busy = false;
+        L.pi->last_used = now_ns();
+    }
     L.pi = nullptr;
     trace_slow("output scatter", tc, (int)rq.size());
     if (L.io) S->io.release(L.io, L.io_bytes);

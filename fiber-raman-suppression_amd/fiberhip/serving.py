@@ -25,8 +25,8 @@ class Server:
 
     memory_budget_bytes bounds the device memory of the server's plans; lanes
     is the number of launches in flight (the process needs lanes + 4 hardware
-    queues, GPU_MAX_HW_QUEUES; on MI355X keep lanes at 12 or below: with 16 and
-    more, long FP32 solves came back non-finite under load); max_batch the
+    queues, GPU_MAX_HW_QUEUES; on MI355X keep lanes at 12 or below: with more
+    lanes, FP32 launches came back non-finite more often under load); max_batch the
     solves per launch.
     Single-mode requests use the adaptive Tsit5 scheme (rtol, atol), multimode
     requests the fixed-step RK4IP with ceil(L / multimode_step_m) steps (at

@@ -382,10 +382,11 @@ int fh_step_sizes(fh_plan plan, int32_t batch, double *hs);
  * of the lanes.  Lanes run on their own streams: give the process lanes + 4
  * hardware queues (GPU_MAX_HW_QUEUES, read when HIP initialises) within the
  * device's compute queues, or lanes sharing a queue run one after another.
- * On MI355X, 12 lanes served a 10,000-request trace with no non-finite
- * solve.  At 16 and 20 lanes the throughput was the same, but long FP32
- * solves of the 1024-thread kernels came back non-finite in 2-3 and about 15
- * launches per 10,000 requests.
+ * On MI355X, launches of the 1024-thread FP32 kernels under many concurrent
+ * lanes occasionally came back non-finite (per 10,000-request trace: 0-5
+ * launches at 12 lanes, 2-3 at 16, about 15 at 20, at equal throughput).
+ * Such a request, if its launch held others, is re-run once alone, and the
+ * plan instance of that launch is released rather than reused.
  * No counterpart exists in the reference (its solves run one at a time in
  * the calling Julia task); callers are batch optimisation drivers and
  * services that issue many solves. */
