@@ -151,7 +151,8 @@ def test_non_finite_request_is_rerun_alone(gpu_context):
     leaves FP32's range in the first Kerr term) fails with FH_ENONFINITE.  If
     it shared a launch it is re-run once, alone, before it is reported
     (attempts 2); the requests launched with it are unaffected -- bitwise equal
-    to the same requests solved alone."""
+    to the same requests solved alone -- and the class is served again after
+    the failed launch's plan was released."""
     import fiberhip
     from fiberhip import grin
 
@@ -163,6 +164,9 @@ def test_non_finite_request_is_rerun_alone(gpu_context):
         good = [srv.wait(ids[i]) for i in (0, 2, 3)]
         with pytest.raises(fiberhip.FiberHIPError, match="FH_ENONFINITE") as err:
             srv.wait(ids[1])
+        # the plan of the failed launch was released: the class is still served
+        again = srv.wait(srv.submit(mm, kind="forward", u0=launches[0], length_m=0.004))
+        assert again["retcode"] == 0 and np.array_equal(again["uL"], good[0]["uL"])
     bad = err.value.result
     assert bad["attempts"] == (2 if _shared_a_launch([g["batch"] for g in good]) else 1)
     assert bad["batch"] == 1
