@@ -433,3 +433,21 @@ def test_bench_serving_report(tmp_path):
     assert r["fp32_vs_fp64_rel_mode_energy_err"]["within_1e-5"] == 7
     assert r["fp32"]["gradients_per_s"] == 3.0 and r["fp64"]["gradients_per_s"] == 1.0
     assert r["fp32_over_fp64_gradients_per_s"] == 3.0
+
+
+def test_grin_setup_backends_are_validated():
+    """solve_for_fiber_modes / overlap_tensor / grin_problem accept "host" and
+    "device" only (the device path is fh_fd_modes / fh_overlap_tensor, GPU)."""
+    from fiberhip import grin
+
+    eps, _, dx = grin.build_grin(1.55, 16, 80.0, 25.0, 0.2, 2.0)
+    with pytest.raises(ValueError):
+        grin.solve_for_fiber_modes(1.55, 2, dx, eps, backend="cpu")
+    with pytest.raises(ValueError):
+        grin.overlap_tensor(np.ones((4, 4, 2)), 1e-6, backend="gpu")
+    # the host backend of the overlap tensor equals the literal four-fold sum
+    rng = np.random.default_rng(4)
+    modes = rng.standard_normal((6, 6, 3))
+    sk = grin.overlap_tensor(modes, 2e-6)
+    lit = np.einsum("xyi,xyj,xyk,xyl->ijkl", modes, modes, modes, modes) / (2e-6) ** 2
+    np.testing.assert_allclose(sk, lit, rtol=1e-12)
